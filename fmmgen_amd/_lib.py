@@ -1,0 +1,89 @@
+"""ctypes binding of libfmm_hip.so (the C ABI in include/fmm_hip.h).
+
+The library is loaded from ``fmmgen_amd/_build`` only; there is no CPU or
+PyTorch fallback.  A missing library raises ``FmmLibraryMissing`` with the
+build command.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfmm_hip.so")
+
+FMM_OK = 0
+MODE_PLAIN_F64 = 0
+MODE_MIXED_F32 = 1
+MODE_MIXED_BF16X3 = 2
+MODES = {"plain": MODE_PLAIN_F64, "f32": MODE_MIXED_F32, "bf16x3": MODE_MIXED_BF16X3}
+
+PHASE_P2M, PHASE_M2M, PHASE_M2L, PHASE_P2P, PHASE_L2L, PHASE_L2P = 1, 2, 4, 8, 16, 32
+PHASE_ALL = 63
+
+OPS = {"P2M": 0, "M2M": 1, "M2L": 2, "L2L": 3, "L2P": 4, "M2P": 5, "P2P": 6, "P2M_S": 7}
+
+# Every symbol include/fmm_hip.h declares.
+EXPORTS = (
+    "fmm_last_error", "fmm_abi_version", "fmm_variant_orders", "fmm_tree_build", "fmm_tree_destroy",
+    "fmm_tree_info", "fmm_tree_set_sources", "fmm_evaluate", "fmm_evaluate_phases", "fmm_direct_sampled",
+    "fmm_op_batch", "fmm_export_cells", "fmm_export_particles", "fmm_export_lists", "fmm_export_expansions",
+)
+
+
+class FmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fmm error {code}: {msg}")
+        self.code = code
+
+
+class FmmLibraryMissing(ImportError):
+    pass
+
+
+_lib = None
+
+vp = ctypes.c_void_p
+sz = ctypes.c_size_t
+ci = ctypes.c_int
+cd = ctypes.c_double
+ll_p = ctypes.POINTER(ctypes.c_longlong)
+d_p = ctypes.POINTER(ctypes.c_double)
+i_p = ctypes.POINTER(ctypes.c_int)
+
+
+def load(path=None):
+    """Loads (once) and returns the library handle."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise FmmLibraryMissing(f"{path} is missing: build it with `python -m fmmgen_amd.build`")
+    L = ctypes.CDLL(path)
+    L.fmm_last_error.restype = ctypes.c_char_p
+    L.fmm_last_error.argtypes = []
+    L.fmm_abi_version.argtypes = []
+    L.fmm_variant_orders.argtypes = [ci, ci, i_p, i_p]
+    L.fmm_tree_build.argtypes = [vp, vp, sz, sz, ci, cd, ci, ci, vp, ctypes.POINTER(vp)]
+    L.fmm_tree_destroy.argtypes = [vp]
+    L.fmm_tree_info.argtypes = [vp, ll_p]
+    L.fmm_tree_set_sources.argtypes = [vp, vp]
+    L.fmm_evaluate.argtypes = [vp, ci, vp, vp]
+    L.fmm_evaluate_phases.argtypes = [vp, ci, ci, vp, vp]
+    L.fmm_direct_sampled.argtypes = [vp, vp, sz, vp, sz, ci, ci, vp, vp]
+    L.fmm_op_batch.argtypes = [ci, ci, ci, ci, sz, vp, vp, ci, vp, ci, vp]
+    L.fmm_export_cells.argtypes = [vp, d_p, ll_p]
+    L.fmm_export_particles.argtypes = [vp, ll_p, ll_p]
+    L.fmm_export_lists.argtypes = [vp, ll_p, ll_p]
+    L.fmm_export_expansions.argtypes = [vp, d_p, d_p]
+    for nm in EXPORTS:
+        if nm not in ("fmm_last_error",):
+            getattr(L, nm).restype = ci
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != FMM_OK:
+        msg = load().fmm_last_error()
+        raise FmmError(code, msg.decode() if msg else "")
+    return code

@@ -1,0 +1,252 @@
+// extern "C" entry points of libfmm_hip.so (declared in include/fmm_hip.h).
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/fmm_hip.h"
+#include "evaluate.h"
+#include "fmm_common.h"
+
+struct fmm_tree_s {
+  fmm::Tree t;
+  const fmm::Variant* v = nullptr;
+};
+
+namespace fmm {
+
+namespace {
+thread_local std::string g_err;
+}
+
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+const Variant* find_variant(int so, int outsz) {
+  if (so == variant_so0_pf.so && outsz == variant_so0_pf.outsz) return &variant_so0_pf;
+  if (so == variant_so1_f.so && outsz == variant_so1_f.outsz) return &variant_so1_f;
+  return nullptr;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return kOk;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return kOutOfMemory;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return kInvalidArgument;
+  }
+}
+
+}  // namespace fmm
+
+using namespace fmm;
+
+extern "C" {
+
+const char* fmm_last_error(void) { return fmm::last_error(); }
+int fmm_abi_version(void) { return 1; }
+
+int fmm_variant_orders(int source_order, int output_size, int* min_order, int* max_order_exclusive) {
+  return guarded([&] {
+    const Variant* v = find_variant(source_order, output_size);
+    FMM_CHECK(v, kUnsupported, "no compiled variant for this source order / output size");
+    if (min_order) *min_order = v->min_order;
+    if (max_order_exclusive) *max_order_exclusive = v->max_order;
+  });
+}
+
+int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncrit, int order, double theta,
+                   int source_order, int output_size, void* stream, fmm_tree_t* out) {
+  return guarded([&] {
+    FMM_CHECK(out, kInvalidArgument, "out is NULL");
+    *out = nullptr;
+    FMM_CHECK(d_pos && d_S, kInvalidArgument, "positions and sources are required");
+    FMM_CHECK(n >= 1 && n < (1u << 31), kInvalidArgument, "particle count must be in [1, 2^31)");
+    FMM_CHECK(ncrit >= 1 && ncrit <= 4096, kInvalidArgument, "ncrit must be in [1, 4096]");
+    FMM_CHECK(theta > 0.0, kInvalidArgument, "theta must be positive");
+    const Variant* v = find_variant(source_order, output_size);
+    FMM_CHECK(v, kUnsupported, "no compiled variant for this source order / output size");
+    FMM_CHECK(order >= v->min_order && order < v->max_order, kUnsupported,
+              "expansion order outside the compiled range [" + std::to_string(v->min_order) + ", " +
+                  std::to_string(v->max_order) + ")");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    auto* h = new fmm_tree_s();
+    try {
+      h->v = v;
+      Tree& t = h->t;
+      t.order = order;
+      t.so = source_order;
+      t.outsz = output_size;
+      t.ncrit = ncrit;
+      t.theta = theta;
+      t.n = n;
+      t.user_S = d_S;
+      build_tree_device(t, d_pos, st);
+      build_lists_device(t, st);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int fmm_tree_destroy(fmm_tree_t tree) {
+  return guarded([&] {
+    if (tree) (void)hipDeviceSynchronize();
+    delete tree;
+  });
+}
+
+int fmm_tree_info(fmm_tree_t tree, long long* info) {
+  return guarded([&] {
+    FMM_CHECK(tree && info, kInvalidArgument, "NULL argument");
+    const Tree& t = tree->t;
+    info[0] = t.ncells;
+    info[1] = t.nleaves;
+    info[2] = t.depth;
+    info[3] = t.n_m2l;
+    info[4] = t.n_p2p;
+    info[5] = (long long)t.n;
+    info[6] = t.order;
+    info[7] = t.so;
+  });
+}
+
+int fmm_tree_set_sources(fmm_tree_t tree, const double* d_S) {
+  return guarded([&] {
+    FMM_CHECK(tree && d_S, kInvalidArgument, "NULL argument");
+    tree->t.user_S = d_S;
+  });
+}
+
+int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(tree && d_F, kInvalidArgument, "NULL argument");
+    FMM_CHECK(mode == FMM_MODE_PLAIN_F64 || mode == FMM_MODE_MIXED_F32 || mode == FMM_MODE_MIXED_BF16X3,
+              kInvalidArgument, "unknown evaluation mode");
+    FMM_CHECK((phase_mask & ~kPhaseAll) == 0, kInvalidArgument, "unknown phase bits");
+    tree->v->evaluate(tree->t, mode, phase_mask, d_F, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
+int fmm_evaluate(fmm_tree_t tree, int mode, double* d_F, void* stream) {
+  return fmm_evaluate_phases(tree, mode, FMM_PHASE_ALL, d_F, stream);
+}
+
+int fmm_direct_sampled(const double* d_pos, const double* d_S, size_t n, const long long* d_targets, size_t nt,
+                       int source_order, int output_size, double* d_F, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(d_pos && d_S && d_F && (d_targets || nt == 0), kInvalidArgument, "NULL argument");
+    FMM_CHECK(n < (1u << 31) && nt < (1u << 31), kInvalidArgument, "sizes must be below 2^31");
+    const Variant* v = find_variant(source_order, output_size);
+    FMM_CHECK(v, kUnsupported, "no compiled variant for this source order / output size");
+    v->direct((int)n, d_pos, d_S, d_targets, (int)nt, d_F, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
+int fmm_op_batch(int op, int order, int source_order, int output_size, size_t n, const double* d_xyz,
+                 const double* d_in, int in_stride, double* d_out, int out_stride, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(op >= FMM_OP_P2M && op <= FMM_OP_P2M_S, kInvalidArgument, "unknown operator id");
+    FMM_CHECK(n == 0 || (d_xyz && d_in && d_out), kInvalidArgument, "NULL argument");
+    FMM_CHECK(n < (1u << 31), kInvalidArgument, "batch too large");
+    FMM_CHECK(in_stride >= 1 && out_stride >= 1, kInvalidArgument, "strides must be positive");
+    const Variant* v = find_variant(source_order, output_size);
+    FMM_CHECK(v, kUnsupported, "no compiled variant for this source order / output size");
+    FMM_CHECK(op == FMM_OP_P2P || (order >= v->min_order && order < v->max_order), kUnsupported,
+              "expansion order outside the compiled range");
+    if (n == 0) return;
+    v->op_batch(op, order, (int)n, d_xyz, d_in, in_stride, d_out, out_stride, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
+int fmm_export_cells(fmm_tree_t tree, double* h_geo, long long* h_meta) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    const Tree& t = tree->t;
+    const int nc = t.ncells;
+    FMM_HIP(hipDeviceSynchronize());
+    if (h_geo) {
+      std::vector<double> a(nc);
+      const double* src[5] = {t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(), t.crmax.get()};
+      for (int k = 0; k < 5; ++k) {
+        FMM_HIP(hipMemcpy(a.data(), src[k], sizeof(double) * nc, hipMemcpyDeviceToHost));
+        for (int c = 0; c < nc; ++c) h_geo[5 * (size_t)c + k] = a[c];
+      }
+    }
+    if (h_meta) {
+      std::vector<int> a(nc);
+      const int* src[7] = {t.parent.get(), t.level.get(), t.pstart.get(), t.pcount.get(),
+                           t.nchild.get(), t.first_child.get(), t.octmask.get()};
+      for (int k = 0; k < 7; ++k) {
+        FMM_HIP(hipMemcpy(a.data(), src[k], sizeof(int) * nc, hipMemcpyDeviceToHost));
+        for (int c = 0; c < nc; ++c) h_meta[8 * (size_t)c + k] = a[c];
+      }
+      for (int c = 0; c < nc; ++c) h_meta[8 * (size_t)c + 7] = 0;
+    }
+  });
+}
+
+int fmm_export_particles(fmm_tree_t tree, long long* h_perm, long long* h_leaf) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    const Tree& t = tree->t;
+    const size_t n = t.n;
+    FMM_HIP(hipDeviceSynchronize());
+    std::vector<int> perm(n), lf(n);
+    FMM_HIP(hipMemcpy(perm.data(), t.perm.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
+    FMM_HIP(hipMemcpy(lf.data(), t.leaf_of.get(), sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (size_t j = 0; j < n; ++j) {
+      if (h_perm) h_perm[j] = perm[j];
+      if (h_leaf) h_leaf[perm[j]] = lf[j];
+    }
+  });
+}
+
+int fmm_export_lists(fmm_tree_t tree, long long* h_m2l, long long* h_p2p) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    const Tree& t = tree->t;
+    FMM_HIP(hipDeviceSynchronize());
+    auto dump = [&](const DevBuf<int>& off, const DevBuf<int>& src, long long m, long long* out, bool src_first) {
+      if (!out) return;
+      std::vector<int> o(t.ncells + 1), s(std::max<long long>(m, 1));
+      FMM_HIP(hipMemcpy(o.data(), off.get(), sizeof(int) * (t.ncells + 1), hipMemcpyDeviceToHost));
+      if (m > 0) FMM_HIP(hipMemcpy(s.data(), src.get(), sizeof(int) * m, hipMemcpyDeviceToHost));
+      for (int c = 0; c < t.ncells; ++c)
+        for (int i = o[c]; i < o[c + 1]; ++i) {
+          out[2 * (size_t)i] = src_first ? s[i] : c;
+          out[2 * (size_t)i + 1] = src_first ? c : s[i];
+        }
+    };
+    dump(t.m2l_off, t.m2l_src, t.n_m2l, h_m2l, true);
+    dump(t.p2p_off, t.p2p_src, t.n_p2p, h_p2p, false);
+  });
+}
+
+int fmm_export_expansions(fmm_tree_t tree, double* h_M, double* h_L) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    const Tree& t = tree->t;
+    FMM_HIP(hipDeviceSynchronize());
+    const size_t ms = (size_t)msize(t.order, t.so) * t.ncells, ls = (size_t)lsize(t.order, t.so) * t.ncells;
+    FMM_CHECK(!h_M || t.M.n >= ms, kInvalidArgument, "no FP64 expansions: evaluate first");
+    FMM_CHECK(!h_L || t.L.n >= ls, kInvalidArgument, "no FP64 expansions: evaluate first");
+    if (h_M) FMM_HIP(hipMemcpy(h_M, t.M.get(), sizeof(double) * ms, hipMemcpyDeviceToHost));
+    if (h_L) FMM_HIP(hipMemcpy(h_L, t.L.get(), sizeof(double) * ls, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

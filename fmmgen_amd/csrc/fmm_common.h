@@ -1,0 +1,137 @@
+// Shared definitions of the MI355X FMM library: error plumbing, the tree
+// handle, and small device helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace fmm {
+
+// ------------------------------------------------------------ status codes
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = 1,
+  kUnsupported = 2,
+  kHipError = 3,
+  kTreeTooDeep = 4,
+  kOutOfMemory = 5,
+};
+
+void set_error(const std::string& msg);
+const char* last_error();
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define FMM_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::fmm::Error{::fmm::kHipError, std::string(#expr) + ": " + hipGetErrorString(e_) + \
+                                             " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"}; \
+  } while (0)
+
+#define FMM_CHECK(cond, code, msg)                  \
+  do {                                              \
+    if (!(cond)) throw ::fmm::Error{(code), (msg)}; \
+  } while (0)
+
+// ------------------------------------------------------------ index layout
+__host__ __device__ constexpr int nterms(int p) { return p < 0 ? 0 : (p + 1) * (p + 2) * (p + 3) / 6; }
+__host__ __device__ constexpr int msize(int p, int so) { return so == 0 ? nterms(p) : nterms(p) - nterms(so - 1); }
+__host__ __device__ constexpr int lsize(int p, int so) { return nterms(p - so); }
+__host__ __device__ constexpr int sourcesize(int so) { return nterms(so) - nterms(so - 1); }
+
+// Device buffer owned by the library (hipMalloc / hipFree).
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    FMM_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    n = count;
+  }
+  T* get() const { return p; }
+};
+
+// Scratch storage of device primitives (rocPRIM temp storage etc.).
+struct Scratch {
+  DevBuf<unsigned char> buf;
+  void* get(size_t bytes) {
+    buf.alloc(bytes);
+    return buf.get();
+  }
+};
+
+// ------------------------------------------------------------ M2L schedule
+// Pairs of one target tile are grouped by (source level, quantised centre
+// offset); every group shares one translation matrix.
+struct M2LSchedule {
+  int ntiles = 0;
+  int ngroups = 0;               // (tile, offset) groups
+  int nkeys = 0;                 // distinct offsets over the whole tree
+  DevBuf<int> tile_cells;        // [ntiles * kTile] target cell or -1
+  DevBuf<int> tile_group_off;    // [ntiles + 1] range in the group list
+  DevBuf<int> group_key;         // [ngroups] distinct-offset id
+  DevBuf<int> group_src;         // [ngroups * kTile] source cell per tile column or -1
+  DevBuf<double> key_dx;         // [nkeys * 3] representative centre offset (target - source)
+  DevBuf<double> key_scale;      // [nkeys] target half-width used for scaling
+};
+
+// ------------------------------------------------------------ tree handle
+struct Tree {
+  // configuration (example/tree.hpp:78-95 + generated macros)
+  int order = 0, so = 0, outsz = 4;
+  size_t ncrit = 0;
+  double theta = 0;
+  size_t n = 0;
+
+  // geometry, canonical numbering: level-major, octant-path order inside a
+  // level; children of a cell are contiguous and follow their parent.
+  int depth = 0;                  // number of levels
+  int ncells = 0, nleaves = 0;
+  std::vector<int> level_off;     // host copy, size depth + 1
+  DevBuf<double> cx, cy, cz, cr, crmax;
+  DevBuf<int> parent, first_child, nchild, octmask, level, pstart, pcount;
+  DevBuf<int> leaves;             // leaf cell ids in canonical order
+
+  // particles in leaf order (particles of every cell are contiguous)
+  DevBuf<int> perm;               // sorted slot -> caller index
+  DevBuf<int> leaf_of;            // sorted slot -> leaf cell
+  DevBuf<double> px, py, pz;      // sorted positions
+  DevBuf<double> pS;              // sorted source strengths [n * SOURCESIZE], per evaluation
+  const double* user_S = nullptr; // borrowed, [n * SOURCESIZE]
+
+  // interaction lists (example/tree.hpp:87-88), multiset-equal to the
+  // reference; CSR by target cell.
+  long long n_m2l = 0, n_p2p = 0;
+  DevBuf<int> m2l_off, m2l_src;   // [ncells+1], [n_m2l]
+  DevBuf<int> p2p_off, p2p_src;   // [ncells+1], [n_p2p]
+  M2LSchedule sched;
+
+  // expansions, [ncells * Msize] and [ncells * Lsize] in FP64
+  DevBuf<double> M, L;
+  // mixed-precision storage (scaled expansions), allocated on demand
+  DevBuf<float> Mf, Lf;
+  DevBuf<double> Fp2p;            // per-target near-field result, sorted order
+  Scratch scratch;
+};
+
+}  // namespace fmm

@@ -1,0 +1,180 @@
+// Order-independent part of one operator variant: source gather, the P2P
+// near field, the FP64 direct sum, order dispatch and the variant table.
+#pragma once
+
+#include "variant_decl.h"
+
+namespace fmm {
+namespace FMM_VARIANT_NS {
+
+__global__ void k_gather_S(int n, const int* __restrict__ perm, const double* __restrict__ S,
+                           double* __restrict__ pS) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int i = perm[j];
+#pragma unroll
+  for (int k = 0; k < SS; ++k) pS[(size_t)j * SS + k] = S[(size_t)i * SS + k];
+}
+
+// P2P near field (example/calculate.cpp:24-40, 208-216): per target
+// particle over the particles of its P2P source leaves, self excluded.
+__global__ void k_p2p_simple(int n, const int* __restrict__ leaf_of, const int* __restrict__ off,
+                             const int* __restrict__ src, const int* __restrict__ pstart,
+                             const int* __restrict__ pcount, const double* __restrict__ px,
+                             const double* __restrict__ py, const double* __restrict__ pz,
+                             const double* __restrict__ pS, double* __restrict__ F) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int A = leaf_of[j];
+  const double xj = px[j], yj = py[j], zj = pz[j];
+  double acc[OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) acc[k] = 0.0;
+  for (int i = off[A]; i < off[A + 1]; ++i) {
+    const int B = src[i];
+    const int b = pstart[B], e = b + pcount[B];
+    for (int s = b; s < e; ++s) {
+      if (s == j) continue;
+      fmmgen_ops::P2P<double>(xj - px[s], yj - py[s], zj - pz[s], pS + (size_t)s * SS, acc);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) F[(size_t)j * OUT + k] = acc[k];
+}
+
+// FP64 direct sum on sampled targets (example/calculate.cpp:252-264): one
+// block per target, sources strided over the lanes, tree reduction in LDS.
+__global__ void k_direct(int n, const double* __restrict__ pos, const double* __restrict__ S,
+                         const long long* __restrict__ tg, int nt, double* __restrict__ F) {
+  const int t = blockIdx.x;
+  if (t >= nt) return;
+  const long long i = tg[t];
+  if (i < 0 || i >= n) {  // out-of-range target: NaN row, no access
+    if (threadIdx.x == 0)
+      for (int k = 0; k < OUT; ++k) F[(size_t)t * OUT + k] = __longlong_as_double(0x7ff8000000000000LL);
+    return;
+  }
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  double acc[OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) acc[k] = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    if (j == i) continue;
+    fmmgen_ops::P2P<double>(xi - pos[3 * j], yi - pos[3 * j + 1], zi - pos[3 * j + 2], S + (size_t)j * SS, acc);
+  }
+  __shared__ double red[256][OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) red[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < OUT; ++k) red[threadIdx.x][k] += red[threadIdx.x + s][k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) F[(size_t)t * OUT + k] = red[0][k];
+}
+
+__global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __restrict__ in, int in_stride,
+                         double* __restrict__ out, int out_stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fmmgen_ops::P2P<double>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], in + (size_t)i * in_stride,
+                          out + (size_t)i * out_stride);
+}
+
+void near_field(Tree& t, int mode, hipStream_t st) {
+  (void)mode;
+  const int n = (int)t.n;
+  k_p2p_simple<<<nblk(n, 128), 128, 0, st>>>(n, t.leaf_of.get(), t.p2p_off.get(), t.p2p_src.get(),
+                                             t.pstart.get(), t.pcount.get(), t.px.get(), t.py.get(), t.pz.get(),
+                                             t.pS.get(), t.Fp2p.get());
+}
+
+template <int P>
+struct OrderTable {
+  static void far(int order, Tree& t, int mode, int mask, hipStream_t st) {
+    if (order == P) far_field_order<P>(t, mode, mask, st);
+    else OrderTable<P + 1>::far(order, t, mode, mask, st);
+  }
+  static void l2p(int order, Tree& t, int mode, int mask, double* F, hipStream_t st) {
+    if (order == P) l2p_final_order<P>(t, mode, mask, F, st);
+    else OrderTable<P + 1>::l2p(order, t, mode, mask, F, st);
+  }
+  static void op(int order, int op, int n, const double* xyz, const double* in, int is, double* out, int os,
+                 hipStream_t st) {
+    if (order == P) op_batch_order<P>(op, n, xyz, in, is, out, os, st);
+    else OrderTable<P + 1>::op(order, op, n, xyz, in, is, out, os, st);
+  }
+};
+template <>
+struct OrderTable<PMAX> {
+  static void far(int, Tree&, int, int, hipStream_t) { throw Error{kUnsupported, "order not compiled"}; }
+  static void l2p(int, Tree&, int, int, double*, hipStream_t) { throw Error{kUnsupported, "order not compiled"}; }
+  static void op(int, int, int, const double*, const double*, int, double*, int, hipStream_t) {
+    throw Error{kUnsupported, "order not compiled"};
+  }
+};
+
+// Tree::compute_field_fmm (example/tree.cpp:243-271): zero F, M and L,
+// then P2M, M2M, {M2L, P2P}, L2L, L2P and the final per-target sum.
+void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
+  const int n = (int)t.n;
+  const int nc = t.ncells;
+  const size_t ms = (size_t)msize(t.order, SO), ls = (size_t)lsize(t.order, SO);
+  t.M.alloc((size_t)nc * ms);
+  t.L.alloc((size_t)nc * ls);
+  t.Fp2p.alloc((size_t)n * OUT);
+  t.pS.alloc((size_t)n * SS);
+  k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+  FMM_HIP(hipMemsetAsync(t.M.get(), 0, sizeof(double) * nc * ms, st));
+  FMM_HIP(hipMemsetAsync(t.L.get(), 0, sizeof(double) * nc * ls, st));
+  OrderTable<PMIN>::far(t.order, t, mode, mask, st);
+  if (mask & kPhaseP2P) near_field(t, mode, st);
+  OrderTable<PMIN>::l2p(t.order, t, mode, mask, F, st);
+}
+
+void op_batch(int op, int order, int n, const double* xyz, const double* in, int in_stride, double* out,
+              int out_stride, hipStream_t st) {
+  // operand sizes of each operator; strides below them would read or
+  // write past an item
+  const int ms = msize(order, SO), ls = lsize(order, SO);
+  int need_in = 0, need_out = 0;
+  switch (op) {
+    case kOpP2M: need_in = 1; need_out = ms; break;
+    case kOpM2M: need_in = ms; need_out = ms; break;
+    case kOpM2L: need_in = ms; need_out = ls; break;
+    case kOpL2L: need_in = ls; need_out = ls; break;
+    case kOpL2P: need_in = ls; need_out = OUT; break;
+    case kOpM2P: need_in = ms; need_out = OUT; break;
+    case kOpP2P: need_in = SS; need_out = OUT; break;
+    case kOpP2MS: need_in = SS; need_out = ms; break;
+    default: throw Error{kInvalidArgument, "unknown operator id"};
+  }
+  FMM_CHECK(in_stride >= need_in && out_stride >= need_out, kInvalidArgument,
+            "operand stride smaller than the operator's array size (in " + std::to_string(need_in) + ", out " +
+                std::to_string(need_out) + ")");
+  if (op == kOpP2P) {
+    k_op_p2p<<<nblk(n, 64), 64, 0, st>>>(n, xyz, in, in_stride, out, out_stride);
+    return;
+  }
+  OrderTable<PMIN>::op(order, op, n, xyz, in, in_stride, out, out_stride, st);
+}
+
+void direct(int n, const double* pos, const double* S, const long long* targets, int nt, double* F,
+            hipStream_t st) {
+  if (nt > 0) k_direct<<<nt, 256, 0, st>>>(n, pos, S, targets, nt, F);
+}
+
+}  // namespace FMM_VARIANT_NS
+
+// The table holds host function pointers; it exists in the host pass only.
+#if !defined(__HIP_DEVICE_COMPILE__)
+const Variant FMM_VARIANT_NAME = {FMM_VARIANT_NS::SO,   FMM_VARIANT_NS::OUT,       FMM_VARIANT_NS::PMIN,
+                                  FMM_VARIANT_NS::PMAX, &FMM_VARIANT_NS::evaluate, &FMM_VARIANT_NS::op_batch,
+                                  &FMM_VARIANT_NS::direct};
+#endif
+
+}  // namespace fmm

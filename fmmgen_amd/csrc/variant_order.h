@@ -1,0 +1,171 @@
+// Per-order kernels of one operator variant: P2M, M2M, M2L, L2L and L2P
+// with the generated operator bodies inlined.  The phase arithmetic follows
+// example/calculate.cpp:135-250:
+//   P2M  per leaf, the example's P2M through M2M (generated P2M_S_<p>),
+//        x = leaf centre - particle
+//   M2M  one level at a time, deepest first, x = parent - child
+//   M2L  per target cell over its M2L sources, x = target - source
+//   L2L  one level at a time, root first, x = child - parent
+//   L2P  per particle, x = particle - leaf centre, added in FP64 to the
+//        near-field sum and written in caller order.
+#pragma once
+
+#include "variant_decl.h"
+
+namespace fmm {
+namespace FMM_VARIANT_NS {
+
+template <int P>
+__global__ void k_p2m(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pstart,
+                      const int* __restrict__ pcount, const double* __restrict__ cx, const double* __restrict__ cy,
+                      const double* __restrict__ cz, const double* __restrict__ px, const double* __restrict__ py,
+                      const double* __restrict__ pz, const double* __restrict__ pS, double* __restrict__ M) {
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nleaves) return;
+  const int c = leaves[li];
+  double* Mc = M + (size_t)c * Op<P>::MS;
+  const double x0 = cx[c], y0 = cy[c], z0 = cz[c];
+  const int b = pstart[c], e = b + pcount[c];
+  for (int j = b; j < e; ++j)
+    fmmgen_ops::P2M_S<double>(x0 - px[j], y0 - py[j], z0 - pz[j], pS + (size_t)j * SS, Mc, P);
+}
+
+template <int P>
+__global__ void k_m2m(int beg, int end, const int* __restrict__ first_child, const int* __restrict__ nchild,
+                      const double* __restrict__ cx, const double* __restrict__ cy, const double* __restrict__ cz,
+                      double* __restrict__ M) {
+  const int p = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= end) return;
+  const int k = nchild[p];
+  if (k == 0) return;
+  const int fc = first_child[p];
+  double* Mp = M + (size_t)p * Op<P>::MS;
+  for (int c = fc; c < fc + k; ++c)
+    fmmgen_ops::M2M<double>(cx[p] - cx[c], cy[p] - cy[c], cz[p] - cz[c], M + (size_t)c * Op<P>::MS, Mp, P);
+}
+
+template <int P>
+__global__ void k_m2l_pairs(int ncells, const int* __restrict__ off, const int* __restrict__ src,
+                            const double* __restrict__ cx, const double* __restrict__ cy,
+                            const double* __restrict__ cz, const double* __restrict__ M, double* __restrict__ L) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= ncells) return;
+  const int b = off[A], e = off[A + 1];
+  double* LA = L + (size_t)A * Op<P>::LS;
+  for (int i = b; i < e; ++i) {
+    const int B = src[i];
+    fmmgen_ops::M2L<double>(cx[A] - cx[B], cy[A] - cy[B], cz[A] - cz[B], M + (size_t)B * Op<P>::MS, LA, P);
+  }
+}
+
+template <int P>
+__global__ void k_l2l(int beg, int end, const int* __restrict__ parent, const double* __restrict__ cx,
+                      const double* __restrict__ cy, const double* __restrict__ cz, double* __restrict__ L) {
+  const int c = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= end) return;
+  const int p = parent[c];
+  fmmgen_ops::L2L<double>(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p], L + (size_t)p * Op<P>::LS,
+                          L + (size_t)c * Op<P>::LS, P);
+}
+
+template <int P>
+__global__ void k_l2p_final(int n, const int* __restrict__ leaf_of, const int* __restrict__ perm,
+                            const double* __restrict__ cx, const double* __restrict__ cy,
+                            const double* __restrict__ cz, const double* __restrict__ px,
+                            const double* __restrict__ py, const double* __restrict__ pz,
+                            const double* __restrict__ L, const double* __restrict__ Fnear, int use_l2p,
+                            double* __restrict__ F) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double acc[OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) acc[k] = Fnear ? Fnear[(size_t)j * OUT + k] : 0.0;
+  if (use_l2p) {
+    const int c = leaf_of[j];
+    fmmgen_ops::L2P<double>(px[j] - cx[c], py[j] - cy[c], pz[j] - cz[c], L + (size_t)c * Op<P>::LS, acc, P);
+  }
+  const int i = perm[j];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) F[(size_t)i * OUT + k] = acc[k];
+}
+
+// One operator over a batch of independent inputs (test entry point).
+template <int P, int OPID>
+__global__ void k_op_batch(int n, const double* __restrict__ xyz, const double* __restrict__ in, int in_stride,
+                           double* __restrict__ out, int out_stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  const double* a = in + (size_t)i * in_stride;
+  double* o = out + (size_t)i * out_stride;
+  if constexpr (OPID == kOpP2M) fmmgen_ops::P2M<double>(x, y, z, a[0], o, P);
+  if constexpr (OPID == kOpM2M) fmmgen_ops::M2M<double>(x, y, z, a, o, P);
+  if constexpr (OPID == kOpM2L) fmmgen_ops::M2L<double>(x, y, z, a, o, P);
+  if constexpr (OPID == kOpL2L) fmmgen_ops::L2L<double>(x, y, z, a, o, P);
+  if constexpr (OPID == kOpL2P) fmmgen_ops::L2P<double>(x, y, z, a, o, P);
+  if constexpr (OPID == kOpM2P) fmmgen_ops::M2P<double>(x, y, z, a, o, P);
+  if constexpr (OPID == kOpP2MS) fmmgen_ops::P2M_S<double>(x, y, z, a, o, P);
+}
+
+template <int P>
+void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
+  (void)mode;
+  const int nc = t.ncells;
+  if (mask & kPhaseP2M)
+    k_p2m<P><<<nblk(t.nleaves, 64), 64, 0, st>>>(t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(),
+                                                 t.cx.get(), t.cy.get(), t.cz.get(), t.px.get(), t.py.get(),
+                                                 t.pz.get(), t.pS.get(), t.M.get());
+  if (mask & kPhaseM2M)
+    for (int l = t.depth - 2; l >= 0; --l) {
+      const int b = t.level_off[l], e = t.level_off[l + 1];
+      k_m2m<P><<<nblk(e - b, 64), 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(),
+                                               t.cy.get(), t.cz.get(), t.M.get());
+    }
+  if (mask & kPhaseM2L)
+    k_m2l_pairs<P><<<nblk(nc, 64), 64, 0, st>>>(nc, t.m2l_off.get(), t.m2l_src.get(), t.cx.get(), t.cy.get(),
+                                                t.cz.get(), t.M.get(), t.L.get());
+  if (mask & kPhaseL2L)
+    for (int l = 1; l < t.depth; ++l) {
+      const int b = t.level_off[l], e = t.level_off[l + 1];
+      k_l2l<P><<<nblk(e - b, 64), 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(),
+                                               t.L.get());
+    }
+}
+
+template <int P>
+void l2p_final_order(Tree& t, int mode, int mask, double* F, hipStream_t st) {
+  (void)mode;
+  const int n = (int)t.n;
+  k_l2p_final<P><<<nblk(n, 128), 128, 0, st>>>(n, t.leaf_of.get(), t.perm.get(), t.cx.get(), t.cy.get(),
+                                               t.cz.get(), t.px.get(), t.py.get(), t.pz.get(), t.L.get(),
+                                               (mask & kPhaseP2P) ? t.Fp2p.get() : nullptr,
+                                               (mask & kPhaseL2P) ? 1 : 0, F);
+}
+
+template <int P>
+void op_batch_order(int op, int n, const double* xyz, const double* in, int is, double* out, int os,
+                    hipStream_t st) {
+  const int g = nblk(n, 64);
+  switch (op) {
+    case kOpP2M: k_op_batch<P, kOpP2M><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
+    case kOpM2M: k_op_batch<P, kOpM2M><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
+    case kOpM2L: k_op_batch<P, kOpM2L><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
+    case kOpL2L: k_op_batch<P, kOpL2L><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
+    case kOpL2P: k_op_batch<P, kOpL2P><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
+    case kOpM2P: k_op_batch<P, kOpM2P><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
+    case kOpP2MS: k_op_batch<P, kOpP2MS><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
+    default: throw Error{kInvalidArgument, "operator has no order"};
+  }
+}
+
+}  // namespace FMM_VARIANT_NS
+}  // namespace fmm
+
+#define FMM_INSTANTIATE_ORDER(P)                                                                        \
+  namespace fmm {                                                                                       \
+  namespace FMM_VARIANT_NS {                                                                            \
+  template void far_field_order<P>(Tree&, int, int, hipStream_t);                                       \
+  template void l2p_final_order<P>(Tree&, int, int, double*, hipStream_t);                              \
+  template void op_batch_order<P>(int, int, const double*, const double*, int, double*, int, hipStream_t); \
+  }                                                                                                     \
+  }

@@ -1,0 +1,175 @@
+"""Host-side mirror of the example's tree / evaluate API, on one GPU.
+
+Reference interface (example/tree.hpp:78-103):
+
+    Tree build_tree(double *pos, double *S, size_t nparticles, size_t ncrit,
+                    size_t order, double theta);
+    void Tree::compute_field_fmm(double *F);
+    void Tree::compute_field_exact(double *F);
+
+Here ``pos``, ``S`` and ``F`` are torch tensors on the GPU (float64,
+C-contiguous): ``pos`` [n, 3], ``S`` [n, SOURCESIZE], ``F`` [n, OUTPUTSIZE].
+As in the reference, the tree borrows ``S`` (read at every evaluation) and
+``compute_field_fmm`` overwrites ``F``.  The work runs in libfmm_hip.so on
+the current torch stream; tensors are device buffers only.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .codegen.layout import Msize, Lsize, sourcesize, outputsize
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _check_dev(t, name, shape1=None):
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError(f"{name} must be a CUDA (HIP) torch tensor")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape1 is not None and (t.dim() != 2 or t.shape[1] != shape1):
+        raise ValueError(f"{name} must have shape [n, {shape1}]")
+
+
+class Tree:
+    """Octree, interaction lists and expansions of one particle set."""
+
+    def __init__(self, handle, pos, S, n, ncrit, order, theta, source_order, potential, field):
+        self._h = handle
+        self.pos, self.S = pos, S
+        self.n, self.ncrit, self.order, self.theta = n, ncrit, order, theta
+        self.source_order = source_order
+        self.potential, self.field = potential, field
+        self.outputsize = outputsize(potential, field)
+        self.msize = Msize(order, source_order)
+        self.lsize = Lsize(order, source_order)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                _lib.load().fmm_tree_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ----------------------------------------------------------- properties
+    def info(self):
+        a = (ctypes.c_longlong * 8)()
+        _lib.check(_lib.load().fmm_tree_info(self._h, a))
+        keys = ("ncells", "nleaves", "levels", "n_m2l", "n_p2p", "n", "order", "source_order")
+        return dict(zip(keys, list(a)))
+
+    def set_sources(self, S):
+        """Same geometry, new source strengths (the tree keeps a pointer)."""
+        _check_dev(S, "S", sourcesize(self.source_order))
+        _lib.check(_lib.load().fmm_tree_set_sources(self._h, _ptr(S)))
+        self.S = S
+
+    # ----------------------------------------------------------- evaluation
+    def compute_field_fmm(self, F, mode="plain", phases=_lib.PHASE_ALL):
+        """Tree::compute_field_fmm (example/tree.cpp:243-271) into F [n, OUTPUTSIZE]."""
+        _check_dev(F, "F", self.outputsize)
+        if F.shape[0] != self.n:
+            raise ValueError("F must have one row per particle")
+        m = _lib.MODES[mode] if isinstance(mode, str) else int(mode)
+        _lib.check(_lib.load().fmm_evaluate_phases(self._h, m, int(phases), _ptr(F), _stream()))
+        return F
+
+    def compute_field_exact(self, F, targets=None):
+        """Tree::compute_field_exact (example/tree.cpp:295-297): FP64 direct sum.
+
+        With ``targets`` (int64 device tensor) only those rows are computed
+        and F has shape [len(targets), OUTPUTSIZE].  The reference adds into a
+        caller-zeroed F; this overwrites it.
+        """
+        if targets is None:
+            targets = torch.arange(self.n, device=self.pos.device, dtype=torch.int64)
+        return direct_sum(self.pos, self.S, targets, F, self.source_order, self.outputsize)
+
+    # ----------------------------------------------------------- exports
+    def export_cells(self):
+        """(geo [ncells,5] = x,y,z,r,rmax; meta [ncells,8] = parent, level, pstart,
+        nleaf, nchild, first_child, octant mask, 0) in canonical numbering."""
+        nc = self.info()["ncells"]
+        geo = np.zeros((nc, 5))
+        meta = np.zeros((nc, 8), dtype=np.int64)
+        _lib.check(_lib.load().fmm_export_cells(
+            self._h, geo.ctypes.data_as(_lib.d_p), meta.ctypes.data_as(_lib.ll_p)))
+        return geo, meta
+
+    def export_particles(self):
+        """(perm [n]: sorted slot -> caller index, leaf [n]: caller index -> leaf)."""
+        perm = np.zeros(self.n, dtype=np.int64)
+        leaf = np.zeros(self.n, dtype=np.int64)
+        _lib.check(_lib.load().fmm_export_particles(
+            self._h, perm.ctypes.data_as(_lib.ll_p), leaf.ctypes.data_as(_lib.ll_p)))
+        return perm, leaf
+
+    def export_lists(self):
+        """(M2L [(source, target)], P2P [(target, source)]) as the reference stores them."""
+        inf = self.info()
+        m2l = np.zeros((inf["n_m2l"], 2), dtype=np.int64)
+        p2p = np.zeros((inf["n_p2p"], 2), dtype=np.int64)
+        _lib.check(_lib.load().fmm_export_lists(
+            self._h, m2l.ctypes.data_as(_lib.ll_p), p2p.ctypes.data_as(_lib.ll_p)))
+        return m2l, p2p
+
+    def export_expansions(self):
+        nc = self.info()["ncells"]
+        M = np.zeros((nc, self.msize))
+        L = np.zeros((nc, self.lsize))
+        _lib.check(_lib.load().fmm_export_expansions(
+            self._h, M.ctypes.data_as(_lib.d_p), L.ctypes.data_as(_lib.d_p)))
+        return M, L
+
+
+def build_tree(pos, S, nparticles, ncrit, order, theta, source_order=0, potential=True, field=True):
+    """build_tree (example/tree.cpp:130-233) on the GPU: tree plus lists."""
+    n = int(nparticles)
+    _check_dev(pos, "pos", 3)
+    _check_dev(S, "S", sourcesize(source_order))
+    if pos.shape[0] != n or S.shape[0] != n:
+        raise ValueError("pos and S must have nparticles rows")
+    h = ctypes.c_void_p()
+    osz = outputsize(potential, field)
+    _lib.check(_lib.load().fmm_tree_build(_ptr(pos), _ptr(S), n, int(ncrit), int(order), float(theta),
+                                          int(source_order), osz, _stream(), ctypes.byref(h)))
+    return Tree(h, pos, S, n, int(ncrit), int(order), float(theta), int(source_order), potential, field)
+
+
+def direct_sum(pos, S, targets, F=None, source_order=0, outsz=4):
+    """FP64 direct sum at ``targets`` over every particle (self excluded)."""
+    _check_dev(pos, "pos", 3)
+    n = pos.shape[0]
+    _check_dev(S, "S", sourcesize(source_order))
+    if targets.dtype != torch.int64 or not targets.is_cuda:
+        raise TypeError("targets must be an int64 CUDA tensor")
+    targets = targets.contiguous()
+    if F is None:
+        F = torch.empty(targets.shape[0], outsz, device=pos.device, dtype=torch.float64)
+    _check_dev(F, "F", outsz)
+    _lib.check(_lib.load().fmm_direct_sampled(_ptr(pos), _ptr(S), n, _ptr(targets), targets.shape[0],
+                                              int(source_order), int(outsz), _ptr(F), _stream()))
+    return F
+
+
+def op_batch(op, order, xyz, inp, out, source_order=0, outsz=4):
+    """Applies one generated device operator to a batch: out[i] += op(xyz[i], inp[i])."""
+    for t, nm in ((xyz, "xyz"), (inp, "inp"), (out, "out")):
+        _check_dev(t, nm)
+    n = xyz.shape[0]
+    _lib.check(_lib.load().fmm_op_batch(_lib.OPS[op], int(order), int(source_order), int(outsz), n,
+                                        _ptr(xyz), _ptr(inp), inp.shape[1], _ptr(out), out.shape[1],
+                                        _stream()))
+    return out

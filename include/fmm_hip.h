@@ -1,0 +1,122 @@
+/* C ABI of the MI355X FMM evaluator (libfmm_hip.so).
+ *
+ * Plain pointers and sizes only.  Pointers named d_* are device memory on
+ * the current HIP device; h_* are host memory.  `stream` is a hipStream_t
+ * (NULL: the default stream).  Every call returns FMM_OK or an error code;
+ * fmm_last_error() describes the last failure of the calling thread.
+ * Calls are stream ordered; only the tree build and the exports
+ * synchronise the stream.
+ *
+ * Reference counterparts (paths in the rpep/fmmgen tree):
+ *   fmm_tree_build        build_tree(pos, S, n, ncrit, order, theta)
+ *                         example/tree.hpp:103, example/tree.cpp:130-233,
+ *                         including interact_dehnen_lazy + sort
+ *                         (example/tree.cpp:217-222)
+ *   fmm_evaluate          Tree::compute_field_fmm(F)
+ *                         example/tree.hpp:89, example/tree.cpp:243-271
+ *   fmm_direct_sampled    Tree::compute_field_exact(F) on a target subset
+ *                         example/tree.hpp:91, example/calculate.cpp:252-264
+ *   fmm_op_batch          the generated operators P2M/M2M/M2L/L2L/L2P/M2P
+ *                         (..., int order) and P2P (...), applied to a
+ *                         batch of independent inputs
+ *                         example/operators.cpp:1376-1489, fmmgen/writer.py:266-311
+ *   fmm_export_*          the Tree members cells, M2L_list, P2P_list, M, L
+ *                         example/tree.hpp:78-95, in canonical numbering
+ */
+#ifndef FMM_HIP_H
+#define FMM_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMM_OK 0
+#define FMM_ERR_INVALID_ARGUMENT 1
+#define FMM_ERR_UNSUPPORTED 2
+#define FMM_ERR_HIP 3
+#define FMM_ERR_TREE_TOO_DEEP 4
+#define FMM_ERR_OUT_OF_MEMORY 5
+
+/* evaluation modes */
+#define FMM_MODE_PLAIN_F64 0    /* every phase in FP64 (the reference arithmetic) */
+#define FMM_MODE_MIXED_F32 1    /* FP32 M2L on MFMA, FP32 + compensated P2P */
+#define FMM_MODE_MIXED_BF16X3 2 /* BF16x3 split M2L on MFMA, FP32 + compensated P2P */
+
+/* phase mask bits for fmm_evaluate_phases */
+#define FMM_PHASE_P2M 1
+#define FMM_PHASE_M2M 2
+#define FMM_PHASE_M2L 4
+#define FMM_PHASE_P2P 8
+#define FMM_PHASE_L2L 16
+#define FMM_PHASE_L2P 32
+#define FMM_PHASE_ALL 63
+
+/* operator ids for fmm_op_batch */
+#define FMM_OP_P2M 0
+#define FMM_OP_M2M 1
+#define FMM_OP_M2L 2
+#define FMM_OP_L2L 3
+#define FMM_OP_L2P 4
+#define FMM_OP_M2P 5
+#define FMM_OP_P2P 6
+#define FMM_OP_P2M_S 7 /* the example's P2M: M2M of the strengths, x = centre - particle */
+
+typedef struct fmm_tree_s* fmm_tree_t;
+
+const char* fmm_last_error(void);
+int fmm_abi_version(void);
+
+/* Compiled operator variant for (source_order, output_size): orders
+ * [*min_order, *max_order_exclusive). */
+int fmm_variant_orders(int source_order, int output_size, int* min_order, int* max_order_exclusive);
+
+/* Builds the octree and the M2L / P2P lists.  d_pos [n*3] interleaved x,y,z;
+ * d_S [n*SOURCESIZE] is borrowed (read at every evaluation) and must stay
+ * valid for the life of the tree, as build_tree keeps pointers into the
+ * caller's arrays (example/tree.cpp:133-137). */
+int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncrit, int order, double theta,
+                   int source_order, int output_size, void* stream, fmm_tree_t* out);
+int fmm_tree_destroy(fmm_tree_t tree);
+
+/* info[0..7] = ncells, nleaves, levels, n_m2l, n_p2p, n, order, source_order */
+int fmm_tree_info(fmm_tree_t tree, long long* info);
+
+/* Rebinds the borrowed source strengths (same geometry, new sources). */
+int fmm_tree_set_sources(fmm_tree_t tree, const double* d_S);
+
+/* Potential / field at every particle, d_F [n*OUTPUTSIZE] in caller order,
+ * overwritten (compute_field_fmm zeroes F first). */
+int fmm_evaluate(fmm_tree_t tree, int mode, double* d_F, void* stream);
+int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, void* stream);
+
+/* FP64 direct sum at d_targets[nt] over all n sources, d_F [nt*OUTPUTSIZE]. */
+int fmm_direct_sampled(const double* d_pos, const double* d_S, size_t n, const long long* d_targets, size_t nt,
+                       int source_order, int output_size, double* d_F, void* stream);
+
+/* Item i: op(d_xyz[3i], d_xyz[3i+1], d_xyz[3i+2], d_in + i*in_stride,
+ * d_out + i*out_stride, order), accumulating into d_out like the generated
+ * operators.  FMM_OP_P2M takes its charge from d_in[i*in_stride]. */
+int fmm_op_batch(int op, int order, int source_order, int output_size, size_t n, const double* d_xyz,
+                 const double* d_in, int in_stride, double* d_out, int out_stride, void* stream);
+
+/* Canonical exports into host memory.
+ * cells: h_geo [ncells*5] = x, y, z, r, rmax; h_meta [ncells*8] = parent,
+ *        level, pstart, nleaf, nchild, first_child, octant mask, 0.
+ * particles: h_perm [n] sorted slot -> caller index, h_leaf [n] caller
+ *        index -> leaf cell.
+ * lists: h_m2l [n_m2l*2] (source, target) as the reference stores them,
+ *        h_p2p [n_p2p*2] (target, source).
+ * expansions of the last evaluation in FP64: h_M [ncells*Msize],
+ *        h_L [ncells*Lsize] (NULL skips). */
+int fmm_export_cells(fmm_tree_t tree, double* h_geo, long long* h_meta);
+int fmm_export_particles(fmm_tree_t tree, long long* h_perm, long long* h_leaf);
+int fmm_export_lists(fmm_tree_t tree, long long* h_m2l, long long* h_p2p);
+int fmm_export_expansions(fmm_tree_t tree, double* h_M, double* h_L);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMM_HIP_H */
