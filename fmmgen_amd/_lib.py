@@ -26,7 +26,9 @@ EXPORTS = (
     "fmm_last_error", "fmm_abi_version", "fmm_variant_orders", "fmm_tree_build", "fmm_tree_destroy",
     "fmm_tree_info", "fmm_tree_set_sources", "fmm_evaluate", "fmm_evaluate_phases", "fmm_direct_sampled",
     "fmm_op_batch", "fmm_export_cells", "fmm_export_particles", "fmm_export_lists", "fmm_export_expansions",
+    "fmm_timing_enable", "fmm_timing_read",
 )
+PHASE_NAMES = ("prep", "P2M", "M2M", "M2L", "P2P", "L2L", "L2P", "unused")
 
 
 class FmmError(RuntimeError):
@@ -75,6 +77,8 @@ def load(path=None):
     L.fmm_export_particles.argtypes = [vp, ll_p, ll_p]
     L.fmm_export_lists.argtypes = [vp, ll_p, ll_p]
     L.fmm_export_expansions.argtypes = [vp, d_p, d_p]
+    L.fmm_timing_enable.argtypes = [vp, ci]
+    L.fmm_timing_read.argtypes = [vp, d_p, ll_p]
     for nm in EXPORTS:
         if nm not in ("fmm_last_error",):
             getattr(L, nm).restype = ci

@@ -20,6 +20,43 @@ thread_local std::string g_err;
 }
 
 void set_error(const std::string& msg) { g_err = msg; }
+
+PhaseTimer::PhaseTimer(Tree& tree, int ph, hipStream_t s) : t(tree), phase(ph), st(s) {
+  if (!t.timing) return;
+  auto take = [&]() {
+    hipEvent_t e;
+    if (!t.event_pool.empty()) {
+      e = t.event_pool.back();
+      t.event_pool.pop_back();
+    } else {
+      FMM_HIP(hipEventCreate(&e));
+    }
+    return e;
+  };
+  a = take();
+  b = take();
+  FMM_HIP(hipEventRecord(a, st));
+}
+
+PhaseTimer::~PhaseTimer() {
+  if (!a) return;
+  (void)hipEventRecord(b, st);
+  t.events.push_back({phase, {a, b}});
+}
+
+// Folds the recorded event pairs into the per-phase totals (synchronises).
+void timing_collect(Tree& t) {
+  for (auto& e : t.events) {
+    FMM_HIP(hipEventSynchronize(e.second.second));
+    float ms = 0.f;
+    FMM_HIP(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+    t.phase_ms[e.first] += ms;
+    t.phase_count[e.first] += 1;
+    t.event_pool.push_back(e.second.first);
+    t.event_pool.push_back(e.second.second);
+  }
+  t.events.clear();
+}
 const char* last_error() { return g_err.c_str(); }
 
 const Variant* find_variant(int so, int outsz) {
@@ -101,8 +138,40 @@ int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncri
 
 int fmm_tree_destroy(fmm_tree_t tree) {
   return guarded([&] {
-    if (tree) (void)hipDeviceSynchronize();
+    if (tree) {
+      (void)hipDeviceSynchronize();
+      for (auto& e : tree->t.events) {
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+      }
+      for (auto e : tree->t.event_pool) (void)hipEventDestroy(e);
+    }
     delete tree;
+  });
+}
+
+int fmm_timing_enable(fmm_tree_t tree, int enable) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    Tree& t = tree->t;
+    timing_collect(t);
+    for (int k = 0; k < kNumPhaseTimers; ++k) {
+      t.phase_ms[k] = 0;
+      t.phase_count[k] = 0;
+    }
+    t.timing = enable != 0;
+  });
+}
+
+int fmm_timing_read(fmm_tree_t tree, double* h_ms, long long* h_count) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    Tree& t = tree->t;
+    timing_collect(t);
+    for (int k = 0; k < kNumPhaseTimers; ++k) {
+      if (h_ms) h_ms[k] = t.phase_ms[k];
+      if (h_count) h_count[k] = t.phase_count[k];
+    }
   });
 }
 
@@ -118,6 +187,8 @@ int fmm_tree_info(fmm_tree_t tree, long long* info) {
     info[5] = (long long)t.n;
     info[6] = t.order;
     info[7] = t.so;
+    info[8] = t.p2p_particle_pairs;
+    info[9] = (long long)t.ncrit;
   });
 }
 

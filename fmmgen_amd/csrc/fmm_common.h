@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace fmm {
@@ -80,6 +81,18 @@ struct Scratch {
   }
 };
 
+// ------------------------------------------------------------ phase timers
+enum PhaseTimerId : int {
+  kTPrep = 0,   // source gather and zeroing
+  kTP2M = 1,
+  kTM2M = 2,
+  kTM2L = 3,
+  kTP2P = 4,
+  kTL2L = 5,
+  kTL2P = 6,    // L2P plus the final per-target sum
+  kNumPhaseTimers = 8,
+};
+
 // ------------------------------------------------------------ M2L schedule
 // Pairs of one target tile are grouped by (source level, quantised centre
 // offset); every group shares one translation matrix.
@@ -132,6 +145,28 @@ struct Tree {
   DevBuf<float> Mf, Lf;
   DevBuf<double> Fp2p;            // per-target near-field result, sorted order
   Scratch scratch;
+
+  // work counters (list build)
+  long long p2p_particle_pairs = 0;  // sum over P2P pairs of nA*nB minus self pairs
+
+  // stream-ordered phase timing (hipEvents recorded around each phase)
+  bool timing = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
+  std::vector<hipEvent_t> event_pool;
+  double phase_ms[kNumPhaseTimers] = {0};
+  long long phase_count[kNumPhaseTimers] = {0};
 };
+
+// RAII helper: records a start/stop event pair for one phase when timing
+// is on.
+struct PhaseTimer {
+  Tree& t;
+  int phase;
+  hipStream_t st;
+  hipEvent_t a = nullptr, b = nullptr;
+  PhaseTimer(Tree& tree, int ph, hipStream_t s);
+  ~PhaseTimer();
+};
+void timing_collect(Tree& t);
 
 }  // namespace fmm

@@ -58,12 +58,21 @@ __device__ __forceinline__ double sqrt_rn(double s) {
 
 __device__ __forceinline__ double rmax_of(double r) { return sqrt_rn((3.0 * r) * r); }
 
+// Sequential mean: the block stages chunks of positions through LDS
+// (coalesced loads), lanes 0..2 add them in index order.
 __global__ void k_mean(const double* __restrict__ pos, long long n, double* __restrict__ out) {
-  const int a = threadIdx.x;
-  if (a >= 3) return;
+  constexpr int kChunk = 1024;
+  __shared__ double buf[3 * kChunk];
   double s = 0.0;
-  for (long long i = 0; i < n; ++i) s = s + pos[3 * i + a];
-  out[a] = s / (double)n;
+  for (long long base = 0; base < n; base += kChunk) {
+    const int m = (int)(n - base < kChunk ? n - base : kChunk);
+    for (int k = threadIdx.x; k < 3 * m; k += blockDim.x) buf[k] = pos[3 * base + k];
+    __syncthreads();
+    if (threadIdx.x < 3)
+      for (int i = 0; i < m; ++i) s = s + buf[3 * i + threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[threadIdx.x] = s / (double)n;
 }
 
 __global__ void k_maxdev(const double* __restrict__ pos, long long n, const double* __restrict__ mean,
@@ -314,6 +323,21 @@ __global__ void k_csr_close(int ncells, int* __restrict__ off, long long m) {
 
 inline int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// P2P particle pairs: sum over list entries of nA * nB, minus self pairs.
+__global__ void k_p2p_work(int ncells, const int* __restrict__ off, const int* __restrict__ src,
+                           const int* __restrict__ pcount, unsigned long long* __restrict__ total) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long w = 0;
+  if (A < ncells) {
+    const long long na = pcount[A];
+    for (int i = off[A]; i < off[A + 1]; ++i) {
+      const int B = src[i];
+      w += (unsigned long long)(na * pcount[B] - (B == A ? na : 0));
+    }
+  }
+  if (w) atomicAdd(total, w);
+}
+
 template <typename T>
 void exclusive_scan_int(Scratch& s, const T* in, T* out, long long n, hipStream_t st) {
   size_t bytes = 0;
@@ -353,7 +377,7 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
   mean.alloc(3);
   mx.alloc(3);
   FMM_HIP(hipMemsetAsync(mx.get(), 0, 3 * sizeof(unsigned long long), st));
-  k_mean<<<1, 64, 0, st>>>(d_pos, n, mean.get());
+  k_mean<<<1, 256, 0, st>>>(d_pos, n, mean.get());
   k_maxdev<<<std::min(blocks(n, 256), 2048), 256, 0, st>>>(d_pos, n, mean.get(), mx.get());
 
   // per-level arrays
@@ -472,6 +496,7 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
     void* tmp = t.scratch.get(bytes);
     FMM_HIP(rocprim::radix_sort_pairs(tmp, bytes, key.get(), k1.get(), v0.get(), t.perm.get(), (size_t)n,
                                       begin_bit, 63u, st));
+    FMM_HIP(hipStreamSynchronize(st));
   }
   t.px.alloc(n); t.py.alloc(n); t.pz.alloc(n); t.leaf_of.alloc(n);
   k_gather_particles<<<blocks(n, 256), 256, 0, st>>>(n, t.perm.get(), d_pos, leaf_orig.get(), t.px.get(),
@@ -496,6 +521,7 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
     t.nleaves = a + b;
     t.leaves.alloc(t.nleaves);
     k_leaf_compact<<<blocks(nc, 256), 256, 0, st>>>(nc, lf.get(), lp.get(), t.leaves.get());
+    FMM_HIP(hipStreamSynchronize(st));
   }
   FMM_HIP(hipStreamSynchronize(st));
 }
@@ -549,6 +575,7 @@ void build_lists_device(Tree& t, hipStream_t st) {
     nxt.alloc(std::max<long long>(add_nxt, 1));
     k_pair_emit<<<blocks(nf, 256), 256, 0, st>>>(nf, cur.get(), off.get(), dec.get(), t.first_child.get(),
                                                  t.nchild.get(), m2l.get(), n_m2l, p2p.get(), n_p2p, nxt.get());
+    FMM_HIP(hipStreamSynchronize(st));
     n_m2l += add_m2l;
     n_p2p += add_p2p;
     FMM_CHECK(add_nxt < (1LL << 31), kOutOfMemory, "traversal frontier exceeds 2^31 pairs");
@@ -561,6 +588,17 @@ void build_lists_device(Tree& t, hipStream_t st) {
   t.n_p2p = n_p2p;
   build_csr(t, m2l, n_m2l, t.m2l_off, t.m2l_src, st);
   build_csr(t, p2p, n_p2p, t.p2p_off, t.p2p_src, st);
+  {
+    DevBuf<unsigned long long> tot;
+    tot.alloc(1);
+    FMM_HIP(hipMemsetAsync(tot.get(), 0, sizeof(unsigned long long), st));
+    k_p2p_work<<<blocks(t.ncells, 256), 256, 0, st>>>(t.ncells, t.p2p_off.get(), t.p2p_src.get(), t.pcount.get(),
+                                                      tot.get());
+    unsigned long long h = 0;
+    FMM_HIP(hipMemcpyAsync(&h, tot.get(), sizeof(h), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    t.p2p_particle_pairs = (long long)h;
+  }
   FMM_HIP(hipStreamSynchronize(st));
 }
 

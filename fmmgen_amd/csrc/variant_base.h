@@ -88,6 +88,7 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
 void near_field(Tree& t, int mode, hipStream_t st) {
   (void)mode;
   const int n = (int)t.n;
+  PhaseTimer pt(t, kTP2P, st);
   k_p2p_simple<<<nblk(n, 128), 128, 0, st>>>(n, t.leaf_of.get(), t.p2p_off.get(), t.p2p_src.get(),
                                              t.pstart.get(), t.pcount.get(), t.px.get(), t.py.get(), t.pz.get(),
                                              t.pS.get(), t.Fp2p.get());
@@ -128,9 +129,12 @@ void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
   t.L.alloc((size_t)nc * ls);
   t.Fp2p.alloc((size_t)n * OUT);
   t.pS.alloc((size_t)n * SS);
-  k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
-  FMM_HIP(hipMemsetAsync(t.M.get(), 0, sizeof(double) * nc * ms, st));
-  FMM_HIP(hipMemsetAsync(t.L.get(), 0, sizeof(double) * nc * ls, st));
+  {
+    PhaseTimer pt(t, kTPrep, st);
+    k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+    FMM_HIP(hipMemsetAsync(t.M.get(), 0, sizeof(double) * nc * ms, st));
+    FMM_HIP(hipMemsetAsync(t.L.get(), 0, sizeof(double) * nc * ls, st));
+  }
   OrderTable<PMIN>::far(t.order, t, mode, mask, st);
   if (mask & kPhaseP2P) near_field(t, mode, st);
   OrderTable<PMIN>::l2p(t.order, t, mode, mask, F, st);

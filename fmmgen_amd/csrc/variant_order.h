@@ -111,31 +111,40 @@ template <int P>
 void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
   (void)mode;
   const int nc = t.ncells;
-  if (mask & kPhaseP2M)
+  if (mask & kPhaseP2M) {
+    PhaseTimer pt(t, kTP2M, st);
     k_p2m<P><<<nblk(t.nleaves, 64), 64, 0, st>>>(t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(),
                                                  t.cx.get(), t.cy.get(), t.cz.get(), t.px.get(), t.py.get(),
                                                  t.pz.get(), t.pS.get(), t.M.get());
-  if (mask & kPhaseM2M)
+  }
+  if (mask & kPhaseM2M) {
+    PhaseTimer pt(t, kTM2M, st);
     for (int l = t.depth - 2; l >= 0; --l) {
       const int b = t.level_off[l], e = t.level_off[l + 1];
       k_m2m<P><<<nblk(e - b, 64), 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(),
                                                t.cy.get(), t.cz.get(), t.M.get());
     }
-  if (mask & kPhaseM2L)
+  }
+  if (mask & kPhaseM2L) {
+    PhaseTimer pt(t, kTM2L, st);
     k_m2l_pairs<P><<<nblk(nc, 64), 64, 0, st>>>(nc, t.m2l_off.get(), t.m2l_src.get(), t.cx.get(), t.cy.get(),
                                                 t.cz.get(), t.M.get(), t.L.get());
-  if (mask & kPhaseL2L)
+  }
+  if (mask & kPhaseL2L) {
+    PhaseTimer pt(t, kTL2L, st);
     for (int l = 1; l < t.depth; ++l) {
       const int b = t.level_off[l], e = t.level_off[l + 1];
       k_l2l<P><<<nblk(e - b, 64), 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(),
                                                t.L.get());
     }
+  }
 }
 
 template <int P>
 void l2p_final_order(Tree& t, int mode, int mask, double* F, hipStream_t st) {
   (void)mode;
   const int n = (int)t.n;
+  PhaseTimer pt(t, kTL2P, st);
   k_l2p_final<P><<<nblk(n, 128), 128, 0, st>>>(n, t.leaf_of.get(), t.perm.get(), t.cx.get(), t.cy.get(),
                                                t.cz.get(), t.px.get(), t.py.get(), t.pz.get(), t.L.get(),
                                                (mask & kPhaseP2P) ? t.Fp2p.get() : nullptr,

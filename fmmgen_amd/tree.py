@@ -65,10 +65,22 @@ class Tree:
 
     # ----------------------------------------------------------- properties
     def info(self):
-        a = (ctypes.c_longlong * 8)()
+        a = (ctypes.c_longlong * 10)()
         _lib.check(_lib.load().fmm_tree_info(self._h, a))
-        keys = ("ncells", "nleaves", "levels", "n_m2l", "n_p2p", "n", "order", "source_order")
+        keys = ("ncells", "nleaves", "levels", "n_m2l", "n_p2p", "n", "order", "source_order",
+                "p2p_particle_pairs", "ncrit")
         return dict(zip(keys, list(a)))
+
+    def timing(self, enable):
+        """Switches stream-ordered per-phase hipEvent timing on or off (resets totals)."""
+        _lib.check(_lib.load().fmm_timing_enable(self._h, int(bool(enable))))
+
+    def timing_read(self):
+        """{phase: (total ms, number of timed evaluations)} since timing was enabled."""
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_longlong * 8)()
+        _lib.check(_lib.load().fmm_timing_read(self._h, ms, cnt))
+        return {nm: (ms[i], cnt[i]) for i, nm in enumerate(_lib.PHASE_NAMES) if nm != "unused"}
 
     def set_sources(self, S):
         """Same geometry, new source strengths (the tree keeps a pointer)."""

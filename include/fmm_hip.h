@@ -80,7 +80,8 @@ int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncri
                    int source_order, int output_size, void* stream, fmm_tree_t* out);
 int fmm_tree_destroy(fmm_tree_t tree);
 
-/* info[0..7] = ncells, nleaves, levels, n_m2l, n_p2p, n, order, source_order */
+/* info[0..9] = ncells, nleaves, levels, n_m2l, n_p2p, n, order,
+ * source_order, P2P particle pairs (self pairs excluded), ncrit */
 int fmm_tree_info(fmm_tree_t tree, long long* info);
 
 /* Rebinds the borrowed source strengths (same geometry, new sources). */
@@ -90,6 +91,16 @@ int fmm_tree_set_sources(fmm_tree_t tree, const double* d_S);
  * overwritten (compute_field_fmm zeroes F first). */
 int fmm_evaluate(fmm_tree_t tree, int mode, double* d_F, void* stream);
 int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, void* stream);
+
+/* Stream-ordered phase timing: with timing on, every evaluation records a
+ * hipEvent pair around each phase on the evaluation stream.  read returns
+ * the totals since the last enable in h_ms[8] / h_count[8], indexed
+ * 0 prep (gather, zeroing), 1 P2M, 2 M2M, 3 M2L, 4 P2P, 5 L2L,
+ * 6 L2P + final sum, 7 unused (synchronises).  No reference counterpart:
+ * the example times the whole evaluation with std::chrono
+ * (example/main.cpp:154-162). */
+int fmm_timing_enable(fmm_tree_t tree, int enable);
+int fmm_timing_read(fmm_tree_t tree, double* h_ms, long long* h_count);
 
 /* FP64 direct sum at d_targets[nt] over all n sources, d_F [nt*OUTPUTSIZE]. */
 int fmm_direct_sampled(const double* d_pos, const double* d_S, size_t n, const long long* d_targets, size_t nt,
