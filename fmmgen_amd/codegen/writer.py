@@ -127,6 +127,15 @@ def device_source(order, potential=True, field=True, source_order=0,
             code += f"    case {o}: {base}_{o}({', '.join(fwd)}); break;\n"
         code += "    default: break;\n  }\n}\n"
         text += code
+        # compile-time order dispatch: only the requested order is instantiated
+        tparams = ", ".join(["typename T"] + [f"typename A_{a}" for a, k in prog.args if k != "scalar"])
+        params = ", ".join(f"T {a}" if k == "scalar" else f"A_{a}&& {a}" for a, k in prog.args)
+        code = f"template <int P, {tparams}>\n__device__ __forceinline__ void {base}_o({params}) {{\n"
+        for j, o in enumerate(orders):
+            kw = "if constexpr" if j == 0 else "else if constexpr"
+            code += f"  {kw} (P == {o}) {base}_{o}<T>({', '.join(fwd)});\n"
+        code += "}\n"
+        text += code
     text += "}  // namespace fmmgen_ops\n"
     return text
 

@@ -128,6 +128,7 @@ int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncri
       t.user_S = d_S;
       build_tree_device(t, d_pos, st);
       build_lists_device(t, st);
+      build_m2l_schedule(t, st);
     } catch (...) {
       delete h;
       throw;

@@ -44,5 +44,6 @@ const Variant* find_variant(int so, int outsz);
 
 void build_tree_device(Tree& t, const double* d_pos, hipStream_t st);
 void build_lists_device(Tree& t, hipStream_t st);
+void build_m2l_schedule(Tree& t, hipStream_t st);
 
 }  // namespace fmm

@@ -96,16 +96,25 @@ enum PhaseTimerId : int {
 // ------------------------------------------------------------ M2L schedule
 // Pairs of one target tile are grouped by (source level, quantised centre
 // offset); every group shares one translation matrix.
+constexpr int kTile = 16;        // target cells per M2L tile (one MFMA column block)
+
 struct M2LSchedule {
   int ntiles = 0;
   int ngroups = 0;               // (tile, offset) groups
-  int nkeys = 0;                 // distinct offsets over the whole tree
+  int nkeys = 0;                 // distinct (target level, source level, offset) keys
   DevBuf<int> tile_cells;        // [ntiles * kTile] target cell or -1
   DevBuf<int> tile_group_off;    // [ntiles + 1] range in the group list
-  DevBuf<int> group_key;         // [ngroups] distinct-offset id
+  DevBuf<int> group_key;         // [ngroups] key id
   DevBuf<int> group_src;         // [ngroups * kTile] source cell per tile column or -1
-  DevBuf<double> key_dx;         // [nkeys * 3] representative centre offset (target - source)
-  DevBuf<double> key_scale;      // [nkeys] target half-width used for scaling
+  DevBuf<double> key_dx;         // [nkeys * 3] centre offset target - source of one pair of the key
+  DevBuf<double> key_r;          // [nkeys] target half-width (scale of the normalised operator)
+  DevBuf<int> key_dlevel;        // [nkeys] target level - source level
+  // per evaluation: the D vector of every key (FP64 plain, normalised FP32 mixed)
+  DevBuf<double> Dtab64;
+  DevBuf<float> Dtab32;
+  // (step, lane) -> D slot table of the GEMM plan, uploaded once per order
+  DevBuf<short> idxtab;
+  int idxtab_order = -1;
 };
 
 // ------------------------------------------------------------ tree handle

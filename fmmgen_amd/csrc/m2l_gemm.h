@@ -1,0 +1,213 @@
+// M2L as offset-grouped GEMMs on the matrix cores.
+//
+// For the pairs of one (tile, key) group the translation is the matrix
+// T[n][m] = D_{n+m}(x), x = target centre - source centre, non-zero only for
+// |n| + |m| <= p (fmmgen/expansions.py:144-172: L_n += sum_m D_{n+m} M_m).
+// One wavefront owns a tile of 16 target cells and computes
+//     L[n][t] += sum_m T[n][m] * M_src(t)[m]        (t = 0..15)
+// with 16x16x4 MFMAs: A = 16 rows of T by 4 multipole slots, B = the same 4
+// slots of the 16 gathered source multipoles, the 16x16 accumulator block
+// holds 16 rows of the tile's local expansions.  Row blocks I and slot
+// slices K whose lowest degrees already exceed p are never issued (the
+// (I, K) plan is built at compile time from the degree layout), and the
+// accumulators stay in registers across all groups of the tile, so each
+// local expansion is written once.
+//
+// A operand: lane l holds T[16 I + l%16][4 K + l/16] = D[idx(n + m)], read
+// from the group's D vector in LDS through a per-(step, lane) index table;
+// B operand: lane l holds M_src(l%16)[4 K + l/16]; accumulator register r of
+// lane l holds target column l%16 and row 16 I + acc_row(l/16, r), where
+// acc_row = 4 (l/16) + r for the f32 MFMA and (l/16) + 4 r for the f64 MFMA
+// (measured on gfx950 with single-element operands).
+//
+// Mixed precision works on normalised expansions (cell half-width r):
+//   Mhat_m = M_m / r^|m|,  Lhat_n = L_n r^(|n|+1),  Dhat_k = D_k r_A^(|k|+1)
+// so Lhat_A,n = sum_m Dhat_{n+m} Mhat_B,m (r_B / r_A)^|m|; r_B / r_A is a
+// power of two (2^(level_A - level_B)) applied exactly with ldexp.
+#pragma once
+
+#include <utility>
+
+#include "fmm_common.h"
+
+namespace fmm {
+
+template <typename F, int... Is>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// total degree of slot j of a source-order-0 array
+__host__ __device__ constexpr int slot_degree(int j) {
+  int d = 0;
+  while (nterms(d) <= j) ++d;
+  return d;
+}
+
+template <int P, int SO>
+struct M2LPlan {
+  static constexpr int LS = lsize(P, SO), MS = msize(P, SO);
+  static constexpr int NI = (LS + 15) / 16, NK = (MS + 3) / 4;
+  static constexpr int OFF = nterms(SO - 1);  // M slot -> full index
+  static constexpr bool valid(int I, int K) {
+    return 16 * I < LS && 4 * K < MS && slot_degree(16 * I) + slot_degree(4 * K + OFF) <= P;
+  }
+  struct Tab {
+    int nI[NK];
+    int base[NK + 1];
+  };
+  // valid row blocks of slice K are a prefix (rows are ordered by degree)
+  static constexpr Tab make() {
+    Tab t{};
+    int s = 0;
+    for (int K = 0; K < NK; ++K) {
+      int c = 0;
+      for (int I = 0; I < NI; ++I)
+        if (valid(I, K)) c = I + 1;
+      t.nI[K] = c;
+      t.base[K] = s;
+      s += c;
+    }
+    t.base[NK] = s;
+    return t;
+  }
+  static constexpr Tab tab = make();
+  static constexpr int NSTEP = tab.base[NK];
+};
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec4<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+
+__device__ __forceinline__ Vec4<double>::type mfma16x16x4(double a, double b, Vec4<double>::type c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ Vec4<float>::type mfma16x16x4(float a, float b, Vec4<float>::type c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Row (within the 16-row block) of accumulator register r of lane group q = l/16.
+template <typename T>
+__device__ __forceinline__ constexpr int acc_row(int q, int r) {
+  return sizeof(T) == 8 ? q + 4 * r : 4 * q + r;
+}
+
+// Blocks are dealt to the 8 XCDs round robin; hand each XCD one contiguous
+// range of tiles so neighbouring tiles (which share sources) share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int per = (nb + 7) / 8;
+  const int x = b % 8, k = b / 8;
+  return x * per + k;
+}
+
+template <int P, int SO, typename T, bool SCALED>
+__global__ __launch_bounds__(64) void k_m2l_gemm(int ntiles, int nblocks, const int* __restrict__ tile_cells,
+                                                 const int* __restrict__ tile_group_off,
+                                                 const int* __restrict__ group_key,
+                                                 const int* __restrict__ group_src,
+                                                 const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
+                                                 const short* __restrict__ idxtab, const T* __restrict__ M,
+                                                 T* __restrict__ L) {
+  using Plan = M2LPlan<P, SO>;
+  constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NK = Plan::NK, NST = Plan::NSTEP;
+  using V4 = typename Vec4<T>::type;
+  __shared__ short sidx[NST * 64];
+  __shared__ T sD[MS];
+  const int lane = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, nblocks);
+  if (tile >= ntiles) return;  // whole (single-wave) block
+  for (int i = lane; i < NST * 64; i += 64) sidx[i] = idxtab[i];
+  V4 acc[NI];
+#pragma unroll
+  for (int I = 0; I < NI; ++I) acc[I] = V4{0, 0, 0, 0};
+  const int col = lane & 15, kk = lane >> 4;
+  const int g0 = tile_group_off[tile], g1 = tile_group_off[tile + 1];
+  for (int g = g0; g < g1; ++g) {
+    const int key = group_key[g];
+    const int src = group_src[(size_t)g * kTile + col];
+    __syncthreads();  // previous group finished reading sD
+    for (int j = lane; j < MS; j += 64) sD[j] = Dtab[(size_t)key * MS + j];
+    T b[NK];
+    const int dl = SCALED ? key_dlevel[key] : 0;
+    static_for<NK>([&](auto K) {
+      constexpr int k0 = 4 * K;
+      const int m = k0 + kk;
+      T v = T(0);
+      if (src >= 0 && m < MS) v = M[(size_t)src * MS + m];
+      if constexpr (SCALED) {
+        constexpr int d0 = slot_degree(k0 + Plan::OFF), d1 = slot_degree(k0 + 1 + Plan::OFF);
+        constexpr int d2 = slot_degree(k0 + 2 + Plan::OFF), d3 = slot_degree(k0 + 3 + Plan::OFF);
+        const int d = kk == 0 ? d0 : (kk == 1 ? d1 : (kk == 2 ? d2 : d3));
+        v = ldexpf(v, dl * d);
+      }
+      b[K] = v;
+    });
+    __syncthreads();  // sD visible to the wave
+    static_for<NK>([&](auto K) {
+      constexpr int nI = Plan::tab.nI[K];
+      static_for<nI>([&](auto I) {
+        constexpr int s = Plan::tab.base[K] + I;
+        const int ix = sidx[s * 64 + lane];
+        const T a = ix >= 0 ? sD[ix] : T(0);
+        acc[I] = mfma16x16x4(a, b[K], acc[I]);
+      });
+    });
+  }
+  const int cell = tile_cells[tile * kTile + col];
+  if (cell < 0) return;
+  T* Lc = L + (size_t)cell * LS;
+  static_for<NI>([&](auto I) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * I + acc_row<T>(kk, r);
+      if (n < LS) Lc[n] += acc[I][r];
+    }
+  });
+}
+
+// Host side: the (step, lane) -> D slot table of the plan (-1: zero entry).
+template <int P, int SO>
+std::vector<short> m2l_index_table() {
+  using Plan = M2LPlan<P, SO>;
+  // multi-index of every slot up to degree 2p
+  std::vector<int> mx, my, mz;
+  for (int d = 0; d <= 2 * P; ++d)
+    for (int x = d; x >= 0; --x)
+      for (int y = d - x; y >= 0; --y) {
+        mx.push_back(x);
+        my.push_back(y);
+        mz.push_back(d - x - y);
+      }
+  auto idx = [](int x, int y, int z) {
+    const int d = x + y + z;
+    const int a = d - x;
+    return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
+  };
+  std::vector<short> t((size_t)Plan::NSTEP * 64, -1);
+  for (int K = 0; K < Plan::NK; ++K)
+    for (int I = 0; I < Plan::tab.nI[K]; ++I) {
+      const int s = Plan::tab.base[K] + I;
+      for (int l = 0; l < 64; ++l) {
+        const int n = 16 * I + (l & 15), m = 4 * K + (l >> 4);
+        if (n >= Plan::LS || m >= Plan::MS) continue;
+        const int mf = m + Plan::OFF;  // full index of the multipole slot
+        const int dn = mx[n] + my[n] + mz[n], dm = mx[mf] + my[mf] + mz[mf];
+        if (dn + dm > P) continue;
+        t[(size_t)s * 64 + l] = (short)(idx(mx[n] + mx[mf], my[n] + my[mf], mz[n] + mz[mf]) - Plan::OFF);
+      }
+    }
+  return t;
+}
+
+}  // namespace fmm

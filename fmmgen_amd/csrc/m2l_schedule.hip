@@ -1,0 +1,258 @@
+// M2L schedule: target tiles and (tile, offset) groups.
+//
+// The M2L list (example/calculate.cpp:190-206, one M2L(A - B, M_B, L_A) per
+// pair) is regrouped so that the translation runs as small GEMMs:
+//   * target cells that have M2L sources are cut into tiles of kTile
+//     consecutive cells of one level (canonical order, so a tile is a
+//     compact block of space);
+//   * a pair's key is (target level, source level, centre offset in units
+//     of the finer half-width).  Centres sit on a dyadic lattice, so all
+//     pairs of one key share one translation matrix T[n][m] = D_{n+m}
+//     (up to the last-bit rounding of the FP64 centres);
+//   * a group is (tile, key): at most one source per tile column (the
+//     source of a target at a given key is unique), so the group is one
+//     16-column operand of the tile's GEMM, and the tile's local expansions
+//     accumulate in registers across its groups.
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "evaluate.h"
+#include "fmm_common.h"
+
+namespace fmm {
+
+namespace {
+
+inline int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+__global__ void k_has_m2l(int b, int e, const int* __restrict__ off, int* __restrict__ flag) {
+  const int c = b + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < e) flag[c - b] = off[c + 1] > off[c] ? 1 : 0;
+}
+
+__global__ void k_assign_tiles(int b, int e, const int* __restrict__ flag, const int* __restrict__ rank,
+                               int tile_base, int* __restrict__ tile_of, int* __restrict__ col_of,
+                               int* __restrict__ tile_cells) {
+  const int c = b + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= e) return;
+  if (!flag[c - b]) return;
+  const int r = rank[c - b];
+  const int t = tile_base + r / kTile, col = r % kTile;
+  tile_of[c] = t;
+  col_of[c] = col;
+  tile_cells[t * kTile + col] = c;
+}
+
+// Per pair: key (target level, source level, quantised offset) and the
+// pair's tile.  Offsets are integer multiples of the finer half-width.
+__global__ void k_pair_key(long long m, const int* __restrict__ off, const int* __restrict__ src, int ncells,
+                           const int* __restrict__ level, const double* __restrict__ cx,
+                           const double* __restrict__ cy, const double* __restrict__ cz,
+                           const double* __restrict__ cr, unsigned long long* __restrict__ key,
+                           int* __restrict__ tgt, int* __restrict__ bad) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  // target of entry i by binary search in the CSR offsets
+  int lo = 0, hi = ncells;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  const int A = lo, B = src[i];
+  tgt[i] = A;
+  const double rf = fmin(cr[A], cr[B]);
+  const long long qx = llrint((cx[A] - cx[B]) / rf);
+  const long long qy = llrint((cy[A] - cy[B]) / rf);
+  const long long qz = llrint((cz[A] - cz[B]) / rf);
+  const long long lim = 1LL << 17;
+  if (qx <= -lim || qx >= lim || qy <= -lim || qy >= lim || qz <= -lim || qz >= lim) atomicOr(bad, 1);
+  key[i] = ((unsigned long long)(level[A] & 31) << 59) | ((unsigned long long)(level[B] & 31) << 54) |
+           ((unsigned long long)((qx + lim) & 0x3ffff) << 36) | ((unsigned long long)((qy + lim) & 0x3ffff) << 18) |
+           (unsigned long long)((qz + lim) & 0x3ffff);
+}
+
+__global__ void k_iota_ll(long long m, int* __restrict__ v) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < m) v[i] = (int)i;
+}
+
+__global__ void k_run_flags64(long long m, const unsigned long long* __restrict__ k, int* __restrict__ flag) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < m) flag[i] = (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
+}
+
+// Key ids in pair order and one representative pair per key.
+__global__ void k_key_ids(long long m, const int* __restrict__ flag, const int* __restrict__ pos,
+                          const int* __restrict__ order, const int* __restrict__ tgt, const int* __restrict__ src,
+                          const int* __restrict__ level, const double* __restrict__ cx,
+                          const double* __restrict__ cy, const double* __restrict__ cz,
+                          const double* __restrict__ cr, int* __restrict__ keyid, double* __restrict__ key_dx,
+                          double* __restrict__ key_r, int* __restrict__ key_dlevel) {
+  const long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int id = pos[j] + flag[j] - 1;  // inclusive rank
+  const int i = order[j];
+  keyid[i] = id;
+  if (flag[j]) {
+    const int A = tgt[i], B = src[i];
+    key_dx[3 * id + 0] = cx[A] - cx[B];
+    key_dx[3 * id + 1] = cy[A] - cy[B];
+    key_dx[3 * id + 2] = cz[A] - cz[B];
+    key_r[id] = cr[A];
+    key_dlevel[id] = level[A] - level[B];
+  }
+}
+
+__global__ void k_group_keys(long long m, const int* __restrict__ tgt, const int* __restrict__ keyid,
+                             const int* __restrict__ tile_of, unsigned long long* __restrict__ comp) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < m) comp[i] = ((unsigned long long)(unsigned)tile_of[tgt[i]] << 32) | (unsigned)keyid[i];
+}
+
+__global__ void k_fill_groups(long long m, const unsigned long long* __restrict__ comp,
+                              const int* __restrict__ order, const int* __restrict__ flag,
+                              const int* __restrict__ pos, const int* __restrict__ tgt,
+                              const int* __restrict__ src, const int* __restrict__ col_of,
+                              int* __restrict__ group_key, int* __restrict__ group_src,
+                              int* __restrict__ tile_first) {
+  const long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int g = pos[j] + flag[j] - 1;
+  const int i = order[j];
+  const unsigned long long c = comp[j];
+  if (flag[j]) {
+    group_key[g] = (int)(c & 0xffffffffu);
+    const int t = (int)(c >> 32);
+    if (j == 0 || (int)(comp[j - 1] >> 32) != t) tile_first[t] = g;
+  }
+  group_src[(long long)g * kTile + col_of[tgt[i]]] = src[i];
+}
+
+__global__ void k_close_offsets(int n, int* __restrict__ off, int total) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int next = total;
+  off[n] = total;
+  for (int c = n - 1; c >= 0; --c) {
+    if (off[c] < 0) off[c] = next;
+    else next = off[c];
+  }
+}
+
+template <typename T>
+void scan_ex(Scratch& s, const T* in, T* out, long long n, hipStream_t st) {
+  size_t bytes = 0;
+  FMM_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), st));
+  void* tmp = s.get(bytes);
+  FMM_HIP(rocprim::exclusive_scan(tmp, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), st));
+}
+
+int read_int(const int* d, hipStream_t st) {
+  int h = 0;
+  FMM_HIP(hipMemcpyAsync(&h, d, sizeof(int), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  return h;
+}
+
+}  // namespace
+
+void build_m2l_schedule(Tree& t, hipStream_t st) {
+  M2LSchedule& S = t.sched;
+  const int nc = t.ncells;
+  const long long m = t.n_m2l;
+  // tiles, one level at a time
+  DevBuf<int> tile_of, col_of, flag, rank;
+  tile_of.alloc(nc);
+  col_of.alloc(nc);
+  std::vector<int> tiles_per_level(t.depth), tile_base(t.depth + 1, 0);
+  {
+    DevBuf<int> fl, rk;
+    for (int l = 0; l < t.depth; ++l) {
+      const int b = t.level_off[l], e = t.level_off[l + 1], k = e - b;
+      fl.alloc(k + 1);
+      rk.alloc(k + 1);
+      k_has_m2l<<<blocks(k, 256), 256, 0, st>>>(b, e, t.m2l_off.get(), fl.get());
+      scan_ex(t.scratch, fl.get(), rk.get(), k, st);
+      const int cnt = read_int(rk.get() + k - 1, st) + read_int(fl.get() + k - 1, st);
+      tiles_per_level[l] = (cnt + kTile - 1) / kTile;
+      tile_base[l + 1] = tile_base[l] + tiles_per_level[l];
+    }
+    S.ntiles = tile_base[t.depth];
+    S.tile_cells.alloc((size_t)std::max(S.ntiles, 1) * kTile);
+    FMM_HIP(hipMemsetAsync(S.tile_cells.get(), 0xff, sizeof(int) * std::max(S.ntiles, 1) * kTile, st));
+    for (int l = 0; l < t.depth; ++l) {
+      const int b = t.level_off[l], e = t.level_off[l + 1], k = e - b;
+      fl.alloc(k + 1);
+      rk.alloc(k + 1);
+      k_has_m2l<<<blocks(k, 256), 256, 0, st>>>(b, e, t.m2l_off.get(), fl.get());
+      scan_ex(t.scratch, fl.get(), rk.get(), k, st);
+      k_assign_tiles<<<blocks(k, 256), 256, 0, st>>>(b, e, fl.get(), rk.get(), tile_base[l], tile_of.get(),
+                                                     col_of.get(), S.tile_cells.get());
+      FMM_HIP(hipStreamSynchronize(st));
+    }
+  }
+  S.tile_group_off.alloc(S.ntiles + 1);
+  if (m == 0) {
+    S.ngroups = 0;
+    S.nkeys = 0;
+    FMM_HIP(hipMemsetAsync(S.tile_group_off.get(), 0, sizeof(int) * (S.ntiles + 1), st));
+    FMM_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  FMM_CHECK(m < (1LL << 31), kOutOfMemory, "more than 2^31 M2L pairs");
+  DevBuf<unsigned long long> key, key_sorted;
+  DevBuf<int> tgt, order0, order, fl, pos, keyid, bad;
+  key.alloc(m); key_sorted.alloc(m); tgt.alloc(m); order0.alloc(m); order.alloc(m);
+  fl.alloc(m); pos.alloc(m); keyid.alloc(m); bad.alloc(1);
+  FMM_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), st));
+  k_pair_key<<<blocks(m, 256), 256, 0, st>>>(m, t.m2l_off.get(), t.m2l_src.get(), nc, t.level.get(), t.cx.get(),
+                                             t.cy.get(), t.cz.get(), t.cr.get(), key.get(), tgt.get(), bad.get());
+  FMM_CHECK(read_int(bad.get(), st) == 0, kUnsupported, "M2L offset beyond 2^17 half-widths (tree too deep)");
+  // distinct keys
+  k_iota_ll<<<blocks(m, 256), 256, 0, st>>>(m, order0.get());
+  {
+    size_t bytes = 0;
+    FMM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key.get(), key_sorted.get(), order0.get(), order.get(),
+                                      (size_t)m, 0, 64, st));
+    void* tmp = t.scratch.get(bytes);
+    FMM_HIP(rocprim::radix_sort_pairs(tmp, bytes, key.get(), key_sorted.get(), order0.get(), order.get(),
+                                      (size_t)m, 0, 64, st));
+  }
+  k_run_flags64<<<blocks(m, 256), 256, 0, st>>>(m, key_sorted.get(), fl.get());
+  scan_ex(t.scratch, fl.get(), pos.get(), m, st);
+  S.nkeys = read_int(pos.get() + m - 1, st) + read_int(fl.get() + m - 1, st);
+  S.key_dx.alloc(3 * (size_t)S.nkeys);
+  S.key_r.alloc(S.nkeys);
+  S.key_dlevel.alloc(S.nkeys);
+  k_key_ids<<<blocks(m, 256), 256, 0, st>>>(m, fl.get(), pos.get(), order.get(), tgt.get(), t.m2l_src.get(),
+                                            t.level.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
+                                            keyid.get(), S.key_dx.get(), S.key_r.get(), S.key_dlevel.get());
+  // groups = runs of (tile, key)
+  k_group_keys<<<blocks(m, 256), 256, 0, st>>>(m, tgt.get(), keyid.get(), tile_of.get(), key.get());
+  k_iota_ll<<<blocks(m, 256), 256, 0, st>>>(m, order0.get());
+  {
+    size_t bytes = 0;
+    FMM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, key.get(), key_sorted.get(), order0.get(), order.get(),
+                                      (size_t)m, 0, 64, st));
+    void* tmp = t.scratch.get(bytes);
+    FMM_HIP(rocprim::radix_sort_pairs(tmp, bytes, key.get(), key_sorted.get(), order0.get(), order.get(),
+                                      (size_t)m, 0, 64, st));
+  }
+  k_run_flags64<<<blocks(m, 256), 256, 0, st>>>(m, key_sorted.get(), fl.get());
+  scan_ex(t.scratch, fl.get(), pos.get(), m, st);
+  S.ngroups = read_int(pos.get() + m - 1, st) + read_int(fl.get() + m - 1, st);
+  S.group_key.alloc(S.ngroups);
+  S.group_src.alloc((size_t)S.ngroups * kTile);
+  FMM_HIP(hipMemsetAsync(S.group_src.get(), 0xff, sizeof(int) * (size_t)S.ngroups * kTile, st));
+  FMM_HIP(hipMemsetAsync(S.tile_group_off.get(), 0xff, sizeof(int) * (S.ntiles + 1), st));
+  k_fill_groups<<<blocks(m, 256), 256, 0, st>>>(m, key_sorted.get(), order.get(), fl.get(), pos.get(), tgt.get(),
+                                                t.m2l_src.get(), col_of.get(), S.group_key.get(), S.group_src.get(),
+                                                S.tile_group_off.get());
+  k_close_offsets<<<1, 1, 0, st>>>(S.ntiles, S.tile_group_off.get(), S.ngroups);
+  FMM_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace fmm

@@ -10,6 +10,7 @@
 //        near-field sum and written in caller order.
 #pragma once
 
+#include "m2l_gemm.h"
 #include "variant_decl.h"
 
 namespace fmm {
@@ -27,7 +28,7 @@ __global__ void k_p2m(int nleaves, const int* __restrict__ leaves, const int* __
   const double x0 = cx[c], y0 = cy[c], z0 = cz[c];
   const int b = pstart[c], e = b + pcount[c];
   for (int j = b; j < e; ++j)
-    fmmgen_ops::P2M_S<double>(x0 - px[j], y0 - py[j], z0 - pz[j], pS + (size_t)j * SS, Mc, P);
+    fmmgen_ops::P2M_S_o<P, double>(x0 - px[j], y0 - py[j], z0 - pz[j], pS + (size_t)j * SS, Mc);
 }
 
 template <int P>
@@ -41,21 +42,55 @@ __global__ void k_m2m(int beg, int end, const int* __restrict__ first_child, con
   const int fc = first_child[p];
   double* Mp = M + (size_t)p * Op<P>::MS;
   for (int c = fc; c < fc + k; ++c)
-    fmmgen_ops::M2M<double>(cx[p] - cx[c], cy[p] - cy[c], cz[p] - cz[c], M + (size_t)c * Op<P>::MS, Mp, P);
+    fmmgen_ops::M2M_o<P, double>(cx[p] - cx[c], cy[p] - cy[c], cz[p] - cz[c], M + (size_t)c * Op<P>::MS, Mp);
+}
+
+// D vector of every M2L key (the derivative block of the generated M2L
+// operator, fmmgen/writer.py:168-182), FP64, or normalised by the target
+// half-width (Dhat_k = D_k r^(|k|+1)) and rounded to FP32.
+template <int P, typename T, bool SCALED>
+__global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const double* __restrict__ key_r,
+                       T* __restrict__ Dtab) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nkeys) return;
+  constexpr int MS = Op<P>::MS;
+  double D[MS];
+  fmmgen_ops::Dgen_o<P, double>(key_dx[3 * k], key_dx[3 * k + 1], key_dx[3 * k + 2], D);
+  if constexpr (SCALED) {
+    const double r = key_r[k];
+    double pw[P + 2];
+    pw[0] = r;
+#pragma unroll
+    for (int d = 1; d <= P + 1; ++d) pw[d] = pw[d - 1] * r;
+#pragma unroll
+    for (int j = 0; j < MS; ++j) D[j] *= pw[slot_degree(j + M2LPlan<P, SO>::OFF)];
+  }
+#pragma unroll
+  for (int j = 0; j < MS; ++j) Dtab[(size_t)k * MS + j] = (T)D[j];
 }
 
 template <int P>
-__global__ void k_m2l_pairs(int ncells, const int* __restrict__ off, const int* __restrict__ src,
-                            const double* __restrict__ cx, const double* __restrict__ cy,
-                            const double* __restrict__ cz, const double* __restrict__ M, double* __restrict__ L) {
-  const int A = blockIdx.x * blockDim.x + threadIdx.x;
-  if (A >= ncells) return;
-  const int b = off[A], e = off[A + 1];
-  double* LA = L + (size_t)A * Op<P>::LS;
-  for (int i = b; i < e; ++i) {
-    const int B = src[i];
-    fmmgen_ops::M2L<double>(cx[A] - cx[B], cy[A] - cy[B], cz[A] - cz[B], M + (size_t)B * Op<P>::MS, LA, P);
-  }
+void ensure_index_table(Tree& t, hipStream_t st) {
+  if (t.sched.idxtab_order == P) return;
+  const std::vector<short> tab = m2l_index_table<P, SO>();
+  t.sched.idxtab.alloc(tab.size());
+  FMM_HIP(hipMemcpyAsync(t.sched.idxtab.get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  t.sched.idxtab_order = P;
+}
+
+template <int P>
+void m2l_plain(Tree& t, hipStream_t st) {
+  M2LSchedule& S = t.sched;
+  if (S.ngroups == 0) return;
+  ensure_index_table<P>(t, st);
+  S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
+  k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
+  const int nb = S.ntiles;
+  const int grid = ((nb + 7) / 8) * 8;
+  k_m2l_gemm<P, SO, double, false><<<grid, 64, 0, st>>>(S.ntiles, grid, S.tile_cells.get(), S.tile_group_off.get(),
+                                                        S.group_key.get(), S.group_src.get(), S.key_dlevel.get(),
+                                                        S.Dtab64.get(), S.idxtab.get(), t.M.get(), t.L.get());
 }
 
 template <int P>
@@ -64,8 +99,8 @@ __global__ void k_l2l(int beg, int end, const int* __restrict__ parent, const do
   const int c = beg + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= end) return;
   const int p = parent[c];
-  fmmgen_ops::L2L<double>(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p], L + (size_t)p * Op<P>::LS,
-                          L + (size_t)c * Op<P>::LS, P);
+  fmmgen_ops::L2L_o<P, double>(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p], L + (size_t)p * Op<P>::LS,
+                               L + (size_t)c * Op<P>::LS);
 }
 
 template <int P>
@@ -82,7 +117,7 @@ __global__ void k_l2p_final(int n, const int* __restrict__ leaf_of, const int* _
   for (int k = 0; k < OUT; ++k) acc[k] = Fnear ? Fnear[(size_t)j * OUT + k] : 0.0;
   if (use_l2p) {
     const int c = leaf_of[j];
-    fmmgen_ops::L2P<double>(px[j] - cx[c], py[j] - cy[c], pz[j] - cz[c], L + (size_t)c * Op<P>::LS, acc, P);
+    fmmgen_ops::L2P_o<P, double>(px[j] - cx[c], py[j] - cy[c], pz[j] - cz[c], L + (size_t)c * Op<P>::LS, acc);
   }
   const int i = perm[j];
 #pragma unroll
@@ -98,13 +133,13 @@ __global__ void k_op_batch(int n, const double* __restrict__ xyz, const double* 
   const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
   const double* a = in + (size_t)i * in_stride;
   double* o = out + (size_t)i * out_stride;
-  if constexpr (OPID == kOpP2M) fmmgen_ops::P2M<double>(x, y, z, a[0], o, P);
-  if constexpr (OPID == kOpM2M) fmmgen_ops::M2M<double>(x, y, z, a, o, P);
-  if constexpr (OPID == kOpM2L) fmmgen_ops::M2L<double>(x, y, z, a, o, P);
-  if constexpr (OPID == kOpL2L) fmmgen_ops::L2L<double>(x, y, z, a, o, P);
-  if constexpr (OPID == kOpL2P) fmmgen_ops::L2P<double>(x, y, z, a, o, P);
-  if constexpr (OPID == kOpM2P) fmmgen_ops::M2P<double>(x, y, z, a, o, P);
-  if constexpr (OPID == kOpP2MS) fmmgen_ops::P2M_S<double>(x, y, z, a, o, P);
+  if constexpr (OPID == kOpP2M) fmmgen_ops::P2M_o<P, double>(x, y, z, a[0], o);
+  if constexpr (OPID == kOpM2M) fmmgen_ops::M2M_o<P, double>(x, y, z, a, o);
+  if constexpr (OPID == kOpM2L) fmmgen_ops::M2L_o<P, double>(x, y, z, a, o);
+  if constexpr (OPID == kOpL2L) fmmgen_ops::L2L_o<P, double>(x, y, z, a, o);
+  if constexpr (OPID == kOpL2P) fmmgen_ops::L2P_o<P, double>(x, y, z, a, o);
+  if constexpr (OPID == kOpM2P) fmmgen_ops::M2P_o<P, double>(x, y, z, a, o);
+  if constexpr (OPID == kOpP2MS) fmmgen_ops::P2M_S_o<P, double>(x, y, z, a, o);
 }
 
 template <int P>
@@ -127,8 +162,7 @@ void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
   }
   if (mask & kPhaseM2L) {
     PhaseTimer pt(t, kTM2L, st);
-    k_m2l_pairs<P><<<nblk(nc, 64), 64, 0, st>>>(nc, t.m2l_off.get(), t.m2l_src.get(), t.cx.get(), t.cy.get(),
-                                                t.cz.get(), t.M.get(), t.L.get());
+    m2l_plain<P>(t, st);
   }
   if (mask & kPhaseL2L) {
     PhaseTimer pt(t, kTL2L, st);
