@@ -144,5 +144,7 @@ class HipPrinter(_Base):
         return (
             "#include <hip/hip_runtime.h>\n"
             "__device__ __forceinline__ double fmmgen_rsqrt(double v) { return rsqrt(v); }\n"
-            "__device__ __forceinline__ float fmmgen_rsqrt(float v) { return rsqrtf(v); }\n"
+            "// FP32: the hardware reciprocal square root (v_rsq_f32, ~1 ulp); inputs\n"
+            "// are squared distances, never denormal\n"
+            "__device__ __forceinline__ float fmmgen_rsqrt(float v) { return __builtin_amdgcn_rsqf(v); }\n"
         )

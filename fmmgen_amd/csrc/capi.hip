@@ -190,6 +190,9 @@ int fmm_tree_info(fmm_tree_t tree, long long* info) {
     info[7] = t.so;
     info[8] = t.p2p_particle_pairs;
     info[9] = (long long)t.ncrit;
+    info[10] = t.sched.ntiles;
+    info[11] = t.sched.ngroups;
+    info[12] = t.sched.nkeys;
   });
 }
 

@@ -115,7 +115,11 @@ struct M2LSchedule {
   // (step, lane) -> D slot table of the GEMM plan, uploaded once per order
   DevBuf<short> idxtab;
   int idxtab_order = -1;
+  // work items: chunks of at most kGroupsPerChunk groups of one tile
+  int nchunks = 0;
+  DevBuf<int> chunk_tile, chunk_g0, chunk_g1;
 };
+constexpr int kGroupsPerChunk = 48;
 
 // ------------------------------------------------------------ tree handle
 struct Tree {
@@ -132,6 +136,7 @@ struct Tree {
   std::vector<int> level_off;     // host copy, size depth + 1
   DevBuf<double> cx, cy, cz, cr, crmax;
   DevBuf<int> parent, first_child, nchild, octmask, level, pstart, pcount;
+  DevBuf<int> octant;             // octant of the cell in its parent (0 for the root)
   DevBuf<int> leaves;             // leaf cell ids in canonical order
 
   // particles in leaf order (particles of every cell are contiguous)
@@ -146,6 +151,10 @@ struct Tree {
   long long n_m2l = 0, n_p2p = 0;
   DevBuf<int> m2l_off, m2l_src;   // [ncells+1], [n_m2l]
   DevBuf<int> p2p_off, p2p_src;   // [ncells+1], [n_p2p]
+  // P2P sources of each target leaf as merged particle ranges (the target
+  // leaf itself excluded), sorted by start: CSR over cells
+  long long n_rng = 0;
+  DevBuf<int> rng_off, rng_start, rng_len;
   M2LSchedule sched;
 
   // expansions, [ncells * Msize] and [ncells * Lsize] in FP64

@@ -10,8 +10,8 @@
 // holds 16 rows of the tile's local expansions.  Row blocks I and slot
 // slices K whose lowest degrees already exceed p are never issued (the
 // (I, K) plan is built at compile time from the degree layout), and the
-// accumulators stay in registers across all groups of the tile, so each
-// local expansion is written once.
+// accumulators stay in registers across a chunk of up to kGroupsPerChunk
+// groups of the tile; the chunks of a tile add their results atomically.
 //
 // A operand: lane l holds T[16 I + l%16][4 K + l/16] = D[idx(n + m)], read
 // from the group's D vector in LDS through a per-(step, lane) index table;
@@ -112,8 +112,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 }
 
 template <int P, int SO, typename T, bool SCALED>
-__global__ __launch_bounds__(64) void k_m2l_gemm(int ntiles, int nblocks, const int* __restrict__ tile_cells,
-                                                 const int* __restrict__ tile_group_off,
+__global__ __launch_bounds__(64) void k_m2l_gemm(int nchunks, int nblocks, const int* __restrict__ tile_cells,
+                                                 const int* __restrict__ chunk_tile,
+                                                 const int* __restrict__ chunk_g0,
+                                                 const int* __restrict__ chunk_g1,
                                                  const int* __restrict__ group_key,
                                                  const int* __restrict__ group_src,
                                                  const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
@@ -125,14 +127,15 @@ __global__ __launch_bounds__(64) void k_m2l_gemm(int ntiles, int nblocks, const 
   __shared__ short sidx[NST * 64];
   __shared__ T sD[MS];
   const int lane = threadIdx.x;
-  const int tile = xcd_tile(blockIdx.x, nblocks);
-  if (tile >= ntiles) return;  // whole (single-wave) block
+  const int chunk = xcd_tile(blockIdx.x, nblocks);
+  if (chunk >= nchunks) return;  // whole (single-wave) block
+  const int tile = chunk_tile[chunk];
   for (int i = lane; i < NST * 64; i += 64) sidx[i] = idxtab[i];
   V4 acc[NI];
 #pragma unroll
   for (int I = 0; I < NI; ++I) acc[I] = V4{0, 0, 0, 0};
   const int col = lane & 15, kk = lane >> 4;
-  const int g0 = tile_group_off[tile], g1 = tile_group_off[tile + 1];
+  const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
   for (int g = g0; g < g1; ++g) {
     const int key = group_key[g];
     const int src = group_src[(size_t)g * kTile + col];
@@ -171,7 +174,7 @@ __global__ __launch_bounds__(64) void k_m2l_gemm(int ntiles, int nblocks, const 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int n = 16 * I + acc_row<T>(kk, r);
-      if (n < LS) Lc[n] += acc[I][r];
+      if (n < LS) atomicAdd(&Lc[n], acc[I][r]);  // chunks of one tile add into the same L
     }
   });
 }

@@ -2,17 +2,17 @@
 //
 // The M2L list (example/calculate.cpp:190-206, one M2L(A - B, M_B, L_A) per
 // pair) is regrouped so that the translation runs as small GEMMs:
-//   * target cells that have M2L sources are cut into tiles of kTile
-//     consecutive cells of one level (canonical order, so a tile is a
-//     compact block of space);
+//   * target cells that have M2L sources are cut into tiles of kTile cells
+//     of one level, ordered by (octant, parent octant) class and then
+//     canonically, so the cells of a tile share most relative offsets;
 //   * a pair's key is (target level, source level, centre offset in units
 //     of the finer half-width).  Centres sit on a dyadic lattice, so all
 //     pairs of one key share one translation matrix T[n][m] = D_{n+m}
 //     (up to the last-bit rounding of the FP64 centres);
 //   * a group is (tile, key): at most one source per tile column (the
 //     source of a target at a given key is unique), so the group is one
-//     16-column operand of the tile's GEMM, and the tile's local expansions
-//     accumulate in registers across its groups.
+//     16-column operand of the tile's GEMM; the tile's group list is cut
+//     into chunks (one wavefront each) whose results add atomically.
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -34,14 +34,27 @@ __global__ void k_has_m2l(int b, int e, const int* __restrict__ off, int* __rest
   if (c < e) flag[c - b] = off[c + 1] > off[c] ? 1 : 0;
 }
 
-__global__ void k_assign_tiles(int b, int e, const int* __restrict__ flag, const int* __restrict__ rank,
-                               int tile_base, int* __restrict__ tile_of, int* __restrict__ col_of,
-                               int* __restrict__ tile_cells) {
+// Tile order of a level's targets: (octant, parent octant) class first,
+// canonical order inside a class.  Away from the domain boundary, cells of
+// one class see the same set of relative M2L offsets, so a tile of them
+// fills its 16 columns for most offsets.  Cells without M2L work sort last.
+__global__ void k_tile_keys(int b, int e, const int* __restrict__ off, const int* __restrict__ octant,
+                            const int* __restrict__ parent, unsigned long long* __restrict__ key,
+                            int* __restrict__ cell) {
   const int c = b + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= e) return;
-  if (!flag[c - b]) return;
-  const int r = rank[c - b];
-  const int t = tile_base + r / kTile, col = r % kTile;
+  const bool has = off[c + 1] > off[c];
+  const unsigned cls = (unsigned)(octant[c] * 8 + octant[parent[c]]);
+  key[c - b] = has ? (((unsigned long long)cls << 32) | (unsigned)c) : ~0ULL;
+  cell[c - b] = c;
+}
+
+__global__ void k_assign_tiles(int cnt, const int* __restrict__ cell_sorted, int tile_base,
+                               int* __restrict__ tile_of, int* __restrict__ col_of, int* __restrict__ tile_cells) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const int c = cell_sorted[i];
+  const int t = tile_base + i / kTile, col = i % kTile;
   tile_of[c] = t;
   col_of[c] = col;
   tile_cells[t * kTile + col] = c;
@@ -183,14 +196,26 @@ void build_m2l_schedule(Tree& t, hipStream_t st) {
     S.ntiles = tile_base[t.depth];
     S.tile_cells.alloc((size_t)std::max(S.ntiles, 1) * kTile);
     FMM_HIP(hipMemsetAsync(S.tile_cells.get(), 0xff, sizeof(int) * std::max(S.ntiles, 1) * kTile, st));
+    DevBuf<unsigned long long> k0, k1;
+    DevBuf<int> c0, c1;
     for (int l = 0; l < t.depth; ++l) {
       const int b = t.level_off[l], e = t.level_off[l + 1], k = e - b;
+      const int cnt = tiles_per_level[l] == 0 ? 0 : -1;
+      if (cnt == 0) continue;
+      k0.alloc(k); k1.alloc(k); c0.alloc(k); c1.alloc(k);
       fl.alloc(k + 1);
       rk.alloc(k + 1);
       k_has_m2l<<<blocks(k, 256), 256, 0, st>>>(b, e, t.m2l_off.get(), fl.get());
       scan_ex(t.scratch, fl.get(), rk.get(), k, st);
-      k_assign_tiles<<<blocks(k, 256), 256, 0, st>>>(b, e, fl.get(), rk.get(), tile_base[l], tile_of.get(),
-                                                     col_of.get(), S.tile_cells.get());
+      const int ncnt = read_int(rk.get() + k - 1, st) + read_int(fl.get() + k - 1, st);
+      k_tile_keys<<<blocks(k, 256), 256, 0, st>>>(b, e, t.m2l_off.get(), t.octant.get(), t.parent.get(), k0.get(),
+                                                  c0.get());
+      size_t bytes = 0;
+      FMM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k0.get(), k1.get(), c0.get(), c1.get(), (size_t)k, 0, 64, st));
+      void* tmp = t.scratch.get(bytes);
+      FMM_HIP(rocprim::radix_sort_pairs(tmp, bytes, k0.get(), k1.get(), c0.get(), c1.get(), (size_t)k, 0, 64, st));
+      k_assign_tiles<<<blocks(ncnt, 256), 256, 0, st>>>(ncnt, c1.get(), tile_base[l], tile_of.get(), col_of.get(),
+                                                        S.tile_cells.get());
       FMM_HIP(hipStreamSynchronize(st));
     }
   }
@@ -253,6 +278,23 @@ void build_m2l_schedule(Tree& t, hipStream_t st) {
                                                 S.tile_group_off.get());
   k_close_offsets<<<1, 1, 0, st>>>(S.ntiles, S.tile_group_off.get(), S.ngroups);
   FMM_HIP(hipStreamSynchronize(st));
+  // work items: runs of at most kGroupsPerChunk groups of one tile
+  std::vector<int> goff(S.ntiles + 1);
+  FMM_HIP(hipMemcpy(goff.data(), S.tile_group_off.get(), sizeof(int) * (S.ntiles + 1), hipMemcpyDeviceToHost));
+  std::vector<int> ct, c0, c1;
+  for (int tl = 0; tl < S.ntiles; ++tl)
+    for (int g = goff[tl]; g < goff[tl + 1]; g += kGroupsPerChunk) {
+      ct.push_back(tl);
+      c0.push_back(g);
+      c1.push_back(std::min(goff[tl + 1], g + kGroupsPerChunk));
+    }
+  S.nchunks = (int)ct.size();
+  S.chunk_tile.alloc(S.nchunks);
+  S.chunk_g0.alloc(S.nchunks);
+  S.chunk_g1.alloc(S.nchunks);
+  FMM_HIP(hipMemcpy(S.chunk_tile.get(), ct.data(), sizeof(int) * S.nchunks, hipMemcpyHostToDevice));
+  FMM_HIP(hipMemcpy(S.chunk_g0.get(), c0.data(), sizeof(int) * S.nchunks, hipMemcpyHostToDevice));
+  FMM_HIP(hipMemcpy(S.chunk_g1.get(), c1.data(), sizeof(int) * S.nchunks, hipMemcpyHostToDevice));
 }
 
 }  // namespace fmm

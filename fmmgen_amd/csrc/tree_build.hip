@@ -137,8 +137,8 @@ __global__ void k_make_children(int nparent, const int* __restrict__ ccount, con
                                 const double* __restrict__ lcz, const double* __restrict__ lr,
                                 double* __restrict__ ncx, double* __restrict__ ncy, double* __restrict__ ncz,
                                 double* __restrict__ nr, int* __restrict__ ncount, int* __restrict__ nparent_local,
-                                int* __restrict__ first_child_local, int* __restrict__ nchild,
-                                int* __restrict__ octmask) {
+                                int* __restrict__ noctant, int* __restrict__ first_child_local,
+                                int* __restrict__ nchild, int* __restrict__ octmask) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nparent) return;
   int first = -1, cnt = 0, mask = 0;
@@ -157,6 +157,7 @@ __global__ void k_make_children(int nparent, const int* __restrict__ ccount, con
     nr[id] = r;
     ncount[id] = k;
     nparent_local[id] = c;
+    noctant[id] = o;
   }
   first_child_local[c] = first;
   nchild[c] = cnt;
@@ -366,7 +367,107 @@ void build_csr(Tree& t, DevBuf<int2>& pairs, long long m, DevBuf<int>& off, DevB
   k_csr_close<<<1, 1, 0, st>>>(t.ncells, off.get(), m);
 }
 
+// Source ranges: one entry per (target, non-self source) pair keyed by
+// (target, source particle start).
+__global__ void k_rng_keys(int ncells, const int* __restrict__ off, const int* __restrict__ src,
+                           const int* __restrict__ pstart, unsigned long long* __restrict__ key,
+                           int* __restrict__ cnt_out, int* __restrict__ pos) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= ncells) return;
+  int w = pos[A];
+  for (int i = off[A]; i < off[A + 1]; ++i) {
+    const int B = src[i];
+    if (B == A) continue;
+    key[w] = ((unsigned long long)(unsigned)A << 32) | (unsigned)pstart[B];
+    cnt_out[w] = B;
+    ++w;
+  }
+}
+
+__global__ void k_rng_count(int ncells, const int* __restrict__ off, const int* __restrict__ src,
+                            int* __restrict__ n) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= ncells) return;
+  int c = 0;
+  for (int i = off[A]; i < off[A + 1]; ++i) c += src[i] != A;
+  n[A] = c;
+}
+
+// run starts: new target, or a gap to the previous source's end
+__global__ void k_rng_flags(long long m, const unsigned long long* __restrict__ key, const int* __restrict__ srcB,
+                            const int* __restrict__ pcount, int* __restrict__ flag) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int f = 1;
+  if (i > 0 && (key[i] >> 32) == (key[i - 1] >> 32)) {
+    const unsigned prev_end = (unsigned)(key[i - 1] & 0xffffffffu) + (unsigned)pcount[srcB[i - 1]];
+    f = ((unsigned)(key[i] & 0xffffffffu) != prev_end);
+  }
+  flag[i] = f;
+}
+
+__global__ void k_rng_fill(long long m, const unsigned long long* __restrict__ key, const int* __restrict__ srcB,
+                           const int* __restrict__ pcount, const int* __restrict__ flag, const int* __restrict__ pos,
+                           int* __restrict__ rstart, int* __restrict__ rlen, int* __restrict__ row_first) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int r = pos[i] + flag[i] - 1;
+  const int A = (int)(key[i] >> 32);
+  if (flag[i]) {
+    rstart[r] = (int)(key[i] & 0xffffffffu);
+    if (i == 0 || (int)(key[i - 1] >> 32) != A) row_first[A] = r;
+  }
+  atomicAdd(&rlen[r], pcount[srcB[i]]);
+}
+
 }  // namespace
+
+// Merged source particle ranges of every P2P target (self excluded).
+void build_p2p_ranges(Tree& t, hipStream_t st) {
+  const int nc = t.ncells;
+  DevBuf<int> cnt, pos;
+  cnt.alloc(nc + 1);
+  pos.alloc(nc + 1);
+  k_rng_count<<<blocks(nc, 256), 256, 0, st>>>(nc, t.p2p_off.get(), t.p2p_src.get(), cnt.get());
+  exclusive_scan_int(t.scratch, cnt.get(), pos.get(), nc, st);
+  int a = 0, b = 0;
+  FMM_HIP(hipMemcpyAsync(&a, cnt.get() + nc - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipMemcpyAsync(&b, pos.get() + nc - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  const long long m = (long long)a + b;
+  t.rng_off.alloc(nc + 1);
+  FMM_HIP(hipMemsetAsync(t.rng_off.get(), 0xff, sizeof(int) * (nc + 1), st));
+  if (m == 0) {
+    t.n_rng = 0;
+    t.rng_start.alloc(1);
+    t.rng_len.alloc(1);
+    k_csr_close<<<1, 1, 0, st>>>(nc, t.rng_off.get(), 0);
+    FMM_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  DevBuf<unsigned long long> k0, k1;
+  DevBuf<int> b0, b1, fl, ps;
+  k0.alloc(m); k1.alloc(m); b0.alloc(m); b1.alloc(m); fl.alloc(m); ps.alloc(m);
+  k_rng_keys<<<blocks(nc, 256), 256, 0, st>>>(nc, t.p2p_off.get(), t.p2p_src.get(), t.pstart.get(), k0.get(),
+                                              b0.get(), pos.get());
+  size_t bytes = 0;
+  FMM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k0.get(), k1.get(), b0.get(), b1.get(), (size_t)m, 0, 64, st));
+  void* tmp = t.scratch.get(bytes);
+  FMM_HIP(rocprim::radix_sort_pairs(tmp, bytes, k0.get(), k1.get(), b0.get(), b1.get(), (size_t)m, 0, 64, st));
+  k_rng_flags<<<blocks(m, 256), 256, 0, st>>>(m, k1.get(), b1.get(), t.pcount.get(), fl.get());
+  exclusive_scan_int(t.scratch, fl.get(), ps.get(), m, st);
+  FMM_HIP(hipMemcpyAsync(&a, fl.get() + m - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipMemcpyAsync(&b, ps.get() + m - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  t.n_rng = (long long)a + b;
+  t.rng_start.alloc(t.n_rng);
+  t.rng_len.alloc(t.n_rng);
+  FMM_HIP(hipMemsetAsync(t.rng_len.get(), 0, sizeof(int) * t.n_rng, st));
+  k_rng_fill<<<blocks(m, 256), 256, 0, st>>>(m, k1.get(), b1.get(), t.pcount.get(), fl.get(), ps.get(),
+                                             t.rng_start.get(), t.rng_len.get(), t.rng_off.get());
+  k_csr_close<<<1, 1, 0, st>>>(nc, t.rng_off.get(), t.n_rng);
+  FMM_HIP(hipStreamSynchronize(st));
+}
 
 // Builds the tree of `t` from caller positions (device, [n*3]).
 void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
@@ -384,7 +485,7 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
   struct Level {
     int n = 0;
     DevBuf<double> cx, cy, cz, r;
-    DevBuf<int> count, parent, first_child, nchild, octmask;
+    DevBuf<int> count, parent, first_child, nchild, octmask, oct;
   };
   std::vector<Level*> lv;
   auto new_level = [&](int m) {
@@ -392,6 +493,8 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
     L->n = m;
     L->cx.alloc(m); L->cy.alloc(m); L->cz.alloc(m); L->r.alloc(m);
     L->count.alloc(m); L->parent.alloc(m); L->first_child.alloc(m); L->nchild.alloc(m); L->octmask.alloc(m);
+    L->oct.alloc(m);
+    FMM_HIP(hipMemsetAsync(L->oct.get(), 0, sizeof(int) * m, st));
     FMM_HIP(hipMemsetAsync(L->nchild.get(), 0, sizeof(int) * m, st));
     FMM_HIP(hipMemsetAsync(L->octmask.get(), 0, sizeof(int) * m, st));
     FMM_HIP(hipMemsetAsync(L->first_child.get(), 0xff, sizeof(int) * m, st));
@@ -444,8 +547,8 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
     Level* nx = new_level(nnext);
     k_make_children<<<blocks(m, 256), 256, 0, st>>>(
         m, ccount.get(), cidx.get(), cur->cx.get(), cur->cy.get(), cur->cz.get(), cur->r.get(), nx->cx.get(),
-        nx->cy.get(), nx->cz.get(), nx->r.get(), nx->count.get(), nx->parent.get(), cur->first_child.get(),
-        cur->nchild.get(), cur->octmask.get());
+        nx->cy.get(), nx->cz.get(), nx->r.get(), nx->count.get(), nx->parent.get(), nx->oct.get(),
+        cur->first_child.get(), cur->nchild.get(), cur->octmask.get());
     k_descend<<<blocks(n, 256), 256, 0, st>>>(n, pcell.get(), poct.get(), cidx.get(), key.get(), level);
     ++level;
   }
@@ -457,7 +560,7 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
   // concatenate levels into canonical global arrays
   t.cx.alloc(nc); t.cy.alloc(nc); t.cz.alloc(nc); t.cr.alloc(nc); t.crmax.alloc(nc);
   t.parent.alloc(nc); t.first_child.alloc(nc); t.nchild.alloc(nc); t.octmask.alloc(nc);
-  t.level.alloc(nc); t.pstart.alloc(nc); t.pcount.alloc(nc);
+  t.level.alloc(nc); t.pstart.alloc(nc); t.pcount.alloc(nc); t.octant.alloc(nc);
   std::vector<int> hlevel(nc);
   for (int l = 0; l < t.depth; ++l) {
     Level* L = lv[l];
@@ -474,6 +577,7 @@ void build_tree_device(Tree& t, const double* d_pos, hipStream_t st) {
     cp(t.first_child.get() + o, L->first_child.get(), sizeof(int) * m);
     cp(t.nchild.get() + o, L->nchild.get(), sizeof(int) * m);
     cp(t.octmask.get() + o, L->octmask.get(), sizeof(int) * m);
+    cp(t.octant.get() + o, L->oct.get(), sizeof(int) * m);
     if (l > 0) k_offset_add<<<blocks(m, 256), 256, 0, st>>>(t.parent.get() + o, m, level_off[l - 1]);
     if (l + 1 < t.depth) k_offset_add<<<blocks(m, 256), 256, 0, st>>>(t.first_child.get() + o, m, level_off[l + 1]);
     for (int c = 0; c < m; ++c) hlevel[o + c] = l;
@@ -588,6 +692,7 @@ void build_lists_device(Tree& t, hipStream_t st) {
   t.n_p2p = n_p2p;
   build_csr(t, m2l, n_m2l, t.m2l_off, t.m2l_src, st);
   build_csr(t, p2p, n_p2p, t.p2p_off, t.p2p_src, st);
+  build_p2p_ranges(t, st);
   {
     DevBuf<unsigned long long> tot;
     tot.alloc(1);

@@ -85,13 +85,142 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
                           out + (size_t)i * out_stride);
 }
 
+// P2P near field, one wavefront per target leaf.
+//
+// The sources of a target leaf are its own particles (self pairs masked)
+// and the merged particle ranges of its other P2P leaves; they are staged
+// 64 at a time through LDS with coalesced loads.  A leaf of nt <= 32
+// particles is replicated over 64 / pow2(nt) lane slices that take
+// alternate sources of each chunk, and the slices are summed at the end.
+//   T = double: FP64 throughout (the reference arithmetic).
+//   T = float : dx = float(x_t - x_s) from FP64 positions (no cancellation
+//               loss for close pairs), the generated P2P body in FP32, and
+//               compensated (Kahan) accumulation of each 64-source chunk's
+//               partial sum into the running total.
+// The result is written in FP64 per sorted slot.
+template <typename T, bool SELF>
+__device__ __forceinline__ void p2p_chunk(int filled, int sl, int nslice, int tslot, double xt, double yt,
+                                          double zt, const double* sx, const double* sy, const double* sz,
+                                          const T* sS, const int* sid, T (&part)[OUT]) {
+  for (int j = sl; j < filled; j += nslice) {
+    T dx, dy, dz;
+    dx = (T)(xt - sx[j]);
+    dy = (T)(yt - sy[j]);
+    dz = (T)(zt - sz[j]);
+    T sj[SS];
+#pragma unroll
+    for (int k = 0; k < SS; ++k) sj[k] = sS[j * SS + k];
+    if (SELF && sid[j] == tslot) {  // self pair: zero source at unit distance
+      dx = T(1);
+      dy = T(0);
+      dz = T(0);
+#pragma unroll
+      for (int k = 0; k < SS; ++k) sj[k] = T(0);
+    }
+    fmmgen_ops::P2P<T>(dx, dy, dz, sj, part);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_p2p_leaf(int nleaves, int nblocks, const int* __restrict__ leaves,
+                                                 const int* __restrict__ pstart, const int* __restrict__ pcount,
+                                                 const int* __restrict__ rng_off, const int* __restrict__ rng_start,
+                                                 const int* __restrict__ rng_len, const double* __restrict__ px,
+                                                 const double* __restrict__ py, const double* __restrict__ pz,
+                                                 const double* __restrict__ pS, double* __restrict__ F) {
+  constexpr bool kKahan = sizeof(T) == 4;
+  __shared__ double sx[64], sy[64], sz[64];
+  __shared__ T sS[64 * SS];
+  __shared__ int sid[64];
+  const int b = blockIdx.x;
+  const int li = (b % 8) * ((nblocks + 7) / 8) + b / 8;  // XCD-contiguous leaf ranges
+  if (li >= nleaves) return;
+  const int A = leaves[li];
+  const int lane = threadIdx.x;
+  const int t0 = pstart[A], nt = pcount[A];
+  int tpad = 1;
+  while (tpad < nt && tpad < 64) tpad <<= 1;
+  const int nslice = 64 / tpad;
+  const int rb = rng_off[A], re = rng_off[A + 1];
+  for (int tb = 0; tb < nt; tb += 64) {
+    const int tl = tb + (lane % tpad), sl = lane / tpad;
+    const bool active = tl < nt;
+    const int tslot = t0 + tl;
+    const double xt = active ? px[tslot] : 0.0, yt = active ? py[tslot] : 0.0, zt = active ? pz[tslot] : 0.0;
+    T sum[OUT], comp[OUT];
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) sum[k] = comp[k] = T(0);
+    // source sequence: the own leaf (virtual range rb - 1, self pairs
+    // masked), then the merged ranges rb .. re - 1
+    int r = rb - 1, o = 0;
+    while (r < re) {
+      const bool has_self = r < rb;
+      int filled = 0;
+      __syncthreads();  // previous chunk consumed
+      while (filled < 64 && r < re) {
+        const int rs = (r < rb) ? t0 : rng_start[r];
+        const int rl = (r < rb) ? nt : rng_len[r];
+        const int take = min(64 - filled, rl - o);
+        if (lane >= filled && lane < filled + take) {
+          const int j = rs + o + (lane - filled);
+          sx[lane] = px[j];
+          sy[lane] = py[j];
+          sz[lane] = pz[j];
+          sid[lane] = (r < rb) ? j : -1;
+#pragma unroll
+          for (int k = 0; k < SS; ++k) sS[lane * SS + k] = (T)pS[(size_t)j * SS + k];
+        }
+        filled += take;
+        o += take;
+        if (o == rl) {
+          ++r;
+          o = 0;
+        }
+      }
+      __syncthreads();  // chunk visible
+      T part[OUT];
+#pragma unroll
+      for (int k = 0; k < OUT; ++k) part[k] = T(0);
+      if (active) {
+        if (has_self)
+          p2p_chunk<T, true>(filled, sl, nslice, tslot, xt, yt, zt, sx, sy, sz, sS, sid, part);
+        else
+          p2p_chunk<T, false>(filled, sl, nslice, tslot, xt, yt, zt, sx, sy, sz, sS, sid, part);
+      }
+#pragma unroll
+      for (int k = 0; k < OUT; ++k) {
+        if constexpr (kKahan) {
+          const T y = part[k] - comp[k];
+          const T s2 = sum[k] + y;
+          comp[k] = (s2 - sum[k]) - y;
+          sum[k] = s2;
+        } else {
+          sum[k] += part[k];
+        }
+      }
+    }
+    // fold the slices of each target into slice 0
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) {
+      double v = kKahan ? (double)sum[k] - (double)comp[k] : (double)sum[k];
+      for (int w = tpad; w < 64; w <<= 1) v += __shfl_xor(v, w, 64);
+      if (active && sl == 0) F[(size_t)tslot * OUT + k] = v;
+    }
+  }
+}
+
 void near_field(Tree& t, int mode, hipStream_t st) {
-  (void)mode;
-  const int n = (int)t.n;
+  const int nl = t.nleaves;
   PhaseTimer pt(t, kTP2P, st);
-  k_p2p_simple<<<nblk(n, 128), 128, 0, st>>>(n, t.leaf_of.get(), t.p2p_off.get(), t.p2p_src.get(),
-                                             t.pstart.get(), t.pcount.get(), t.px.get(), t.py.get(), t.pz.get(),
-                                             t.pS.get(), t.Fp2p.get());
+  const int grid = ((nl + 7) / 8) * 8;
+  if (mode == 0)
+    k_p2p_leaf<double><<<grid, 64, 0, st>>>(nl, grid, t.leaves.get(), t.pstart.get(), t.pcount.get(),
+                                            t.rng_off.get(), t.rng_start.get(), t.rng_len.get(), t.px.get(),
+                                            t.py.get(), t.pz.get(), t.pS.get(), t.Fp2p.get());
+  else
+    k_p2p_leaf<float><<<grid, 64, 0, st>>>(nl, grid, t.leaves.get(), t.pstart.get(), t.pcount.get(),
+                                           t.rng_off.get(), t.rng_start.get(), t.rng_len.get(), t.px.get(),
+                                           t.py.get(), t.pz.get(), t.pS.get(), t.Fp2p.get());
 }
 
 template <int P>

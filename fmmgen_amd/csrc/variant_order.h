@@ -86,11 +86,11 @@ void m2l_plain(Tree& t, hipStream_t st) {
   ensure_index_table<P>(t, st);
   S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
-  const int nb = S.ntiles;
-  const int grid = ((nb + 7) / 8) * 8;
-  k_m2l_gemm<P, SO, double, false><<<grid, 64, 0, st>>>(S.ntiles, grid, S.tile_cells.get(), S.tile_group_off.get(),
-                                                        S.group_key.get(), S.group_src.get(), S.key_dlevel.get(),
-                                                        S.Dtab64.get(), S.idxtab.get(), t.M.get(), t.L.get());
+  const int grid = ((S.nchunks + 7) / 8) * 8;
+  k_m2l_gemm<P, SO, double, false><<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(),
+                                                        S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
+                                                        S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(),
+                                                        S.idxtab.get(), t.M.get(), t.L.get());
 }
 
 template <int P>

@@ -157,3 +157,32 @@ def test_direct_sampled_matches_oracle(gpu):
     Fo = O.direct_targets(pos, q, tg)
     Fg = direct_sum(_dev(pos), _dev(q), torch.from_numpy(tg).cuda()).cpu().numpy()
     assert util.rel_norm(Fg, Fo) < TOL
+
+
+# ------------------------------------------------------------------ mixed precision
+
+MIXED_CASES = [
+    ("uniform", 20000, 64, 8, 0.5, 0, "f32"),
+    ("plummer", 20000, 32, 6, 0.4, 0, "f32"),
+    ("plummer", 20000, 32, 6, 0.4, 1, "f32"),
+]
+
+
+@pytest.mark.parametrize("kind,n,ncrit,order,theta,so,mode", MIXED_CASES)
+def test_fmm_mixed_close_to_plain(gpu, kind, n, ncrit, order, theta, so, mode):
+    """Mixed modes stay within 1e-6 (relative, norm-wise) of the FP64 result:
+    far below the truncation error of the expansions at these orders."""
+    from fmmgen_amd import build_tree
+    _log(f"mixed {mode} {kind} n={n} p={order} so={so}")
+    pos = _positions(kind, n, 13)
+    S = np.random.default_rng(14).uniform(-1, 1, (n, 1)) if so == 0 else util.dipoles(n, 14)
+    outsz = 4 if so == 0 else 3
+    gt = build_tree(_dev(pos), _dev(S), n, ncrit, order, theta, source_order=so, potential=(so == 0))
+    F64 = torch.zeros(n, outsz, dtype=torch.float64, device="cuda")
+    gt.compute_field_fmm(F64, mode="plain")
+    Fm = torch.zeros_like(F64)
+    gt.compute_field_fmm(Fm, mode=mode)
+    fo = 1 if so == 0 else 0
+    err = util.rel_norm(Fm.cpu().numpy()[:, fo:], F64.cpu().numpy()[:, fo:])
+    _log(f"  mixed vs plain field difference {err:.3e}")
+    assert err < 1e-6
