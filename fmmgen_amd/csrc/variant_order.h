@@ -16,33 +16,80 @@
 namespace fmm {
 namespace FMM_VARIANT_NS {
 
+// Output proxy of a generated body that keeps slots [LO, HI) in registers.
+// Inlined with constant slot indices, every output outside the chunk folds
+// to dead code, so one instantiation per chunk computes only its slots.
+template <typename T, int LO, int HI>
+struct ChunkAcc {
+  T v[HI - LO];
+  struct Ref {
+    T* p;
+    __device__ __forceinline__ void operator+=(T x) {
+      if (p) *p += x;
+    }
+  };
+  __device__ __forceinline__ Ref operator[](int k) { return Ref{(k >= LO && k < HI) ? &v[k - LO] : nullptr}; }
+};
+
+constexpr int kChunk = 32;  // expansion slots per thread
+
+template <int NCH, typename F>
+__device__ __forceinline__ void dispatch_chunk(int ch, F&& f) {
+  static_for<NCH>([&](auto C) {
+    if (ch == C) f(C);
+  });
+}
+
+// P2M: the example's P2M through M2M (example/calculate.cpp:135-161),
+// x = leaf centre - particle, one thread per (leaf, slot chunk).
 template <int P>
-__global__ void k_p2m(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pstart,
+__global__ __launch_bounds__(64) void k_p2m(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pstart,
                       const int* __restrict__ pcount, const double* __restrict__ cx, const double* __restrict__ cy,
                       const double* __restrict__ cz, const double* __restrict__ px, const double* __restrict__ py,
                       const double* __restrict__ pz, const double* __restrict__ pS, double* __restrict__ M) {
+  constexpr int MS = Op<P>::MS, NCH = (MS + kChunk - 1) / kChunk;
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= nleaves) return;
   const int c = leaves[li];
-  double* Mc = M + (size_t)c * Op<P>::MS;
   const double x0 = cx[c], y0 = cy[c], z0 = cz[c];
   const int b = pstart[c], e = b + pcount[c];
-  for (int j = b; j < e; ++j)
-    fmmgen_ops::P2M_S_o<P, double>(x0 - px[j], y0 - py[j], z0 - pz[j], pS + (size_t)j * SS, Mc);
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < MS) ? LO + kChunk : MS;
+    ChunkAcc<double, LO, HI> acc;
+#pragma unroll
+    for (int k = 0; k < HI - LO; ++k) acc.v[k] = 0.0;
+    for (int j = b; j < e; ++j)
+      fmmgen_ops::P2M_S_o<P, double>(x0 - px[j], y0 - py[j], z0 - pz[j], pS + (size_t)j * SS, acc);
+    double* Mc = M + (size_t)c * MS;
+#pragma unroll
+    for (int k = 0; k < HI - LO; ++k) Mc[LO + k] = acc.v[k];
+  });
 }
 
+// M2M (example/calculate.cpp:163-187), x = parent - child, one thread per
+// (parent, slot chunk) of one level; children are summed in registers.
 template <int P>
-__global__ void k_m2m(int beg, int end, const int* __restrict__ first_child, const int* __restrict__ nchild,
+__global__ __launch_bounds__(64) void k_m2m(int beg, int end, const int* __restrict__ first_child, const int* __restrict__ nchild,
                       const double* __restrict__ cx, const double* __restrict__ cy, const double* __restrict__ cz,
                       double* __restrict__ M) {
+  constexpr int MS = Op<P>::MS, NCH = (MS + kChunk - 1) / kChunk;
   const int p = beg + blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= end) return;
   const int k = nchild[p];
   if (k == 0) return;
   const int fc = first_child[p];
-  double* Mp = M + (size_t)p * Op<P>::MS;
-  for (int c = fc; c < fc + k; ++c)
-    fmmgen_ops::M2M_o<P, double>(cx[p] - cx[c], cy[p] - cy[c], cz[p] - cz[c], M + (size_t)c * Op<P>::MS, Mp);
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < MS) ? LO + kChunk : MS;
+    ChunkAcc<double, LO, HI> acc;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.0;
+    for (int c = fc; c < fc + k; ++c)
+      fmmgen_ops::M2M_o<P, double>(cx[p] - cx[c], cy[p] - cy[c], cz[p] - cz[c],
+                                   static_cast<const double*>(M + (size_t)c * MS), acc);
+    double* Mp = M + (size_t)p * MS;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) Mp[LO + j] = acc.v[j];
+  });
 }
 
 // D vector of every M2L key (the derivative block of the generated M2L
@@ -93,18 +140,30 @@ void m2l_plain(Tree& t, hipStream_t st) {
                                                         S.idxtab.get(), t.M.get(), t.L.get());
 }
 
+// L2L (example/calculate.cpp:219-234), x = child - parent, one thread per
+// (child, slot chunk) of one level.
 template <int P>
-__global__ void k_l2l(int beg, int end, const int* __restrict__ parent, const double* __restrict__ cx,
+__global__ __launch_bounds__(64) void k_l2l(int beg, int end, const int* __restrict__ parent, const double* __restrict__ cx,
                       const double* __restrict__ cy, const double* __restrict__ cz, double* __restrict__ L) {
+  constexpr int LS = Op<P>::LS, NCH = (LS + kChunk - 1) / kChunk;
   const int c = beg + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= end) return;
   const int p = parent[c];
-  fmmgen_ops::L2L_o<P, double>(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p], L + (size_t)p * Op<P>::LS,
-                               L + (size_t)c * Op<P>::LS);
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < LS) ? LO + kChunk : LS;
+    ChunkAcc<double, LO, HI> acc;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.0;
+    fmmgen_ops::L2L_o<P, double>(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p],
+                                 static_cast<const double*>(L + (size_t)p * LS), acc);
+    double* Lc = L + (size_t)c * LS;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) Lc[LO + j] += acc.v[j];
+  });
 }
 
 template <int P>
-__global__ void k_l2p_final(int n, const int* __restrict__ leaf_of, const int* __restrict__ perm,
+__global__ __launch_bounds__(64) void k_l2p_final(int n, const int* __restrict__ leaf_of, const int* __restrict__ perm,
                             const double* __restrict__ cx, const double* __restrict__ cy,
                             const double* __restrict__ cz, const double* __restrict__ px,
                             const double* __restrict__ py, const double* __restrict__ pz,
@@ -148,7 +207,7 @@ void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
   const int nc = t.ncells;
   if (mask & kPhaseP2M) {
     PhaseTimer pt(t, kTP2M, st);
-    k_p2m<P><<<nblk(t.nleaves, 64), 64, 0, st>>>(t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(),
+    k_p2m<P><<<dim3(nblk(t.nleaves, 64), (Op<P>::MS + kChunk - 1) / kChunk), 64, 0, st>>>(t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(),
                                                  t.cx.get(), t.cy.get(), t.cz.get(), t.px.get(), t.py.get(),
                                                  t.pz.get(), t.pS.get(), t.M.get());
   }
@@ -156,7 +215,7 @@ void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
     PhaseTimer pt(t, kTM2M, st);
     for (int l = t.depth - 2; l >= 0; --l) {
       const int b = t.level_off[l], e = t.level_off[l + 1];
-      k_m2m<P><<<nblk(e - b, 64), 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(),
+      k_m2m<P><<<dim3(nblk(e - b, 64), (Op<P>::MS + kChunk - 1) / kChunk), 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(),
                                                t.cy.get(), t.cz.get(), t.M.get());
     }
   }
@@ -168,7 +227,7 @@ void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
     PhaseTimer pt(t, kTL2L, st);
     for (int l = 1; l < t.depth; ++l) {
       const int b = t.level_off[l], e = t.level_off[l + 1];
-      k_l2l<P><<<nblk(e - b, 64), 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(),
+      k_l2l<P><<<dim3(nblk(e - b, 64), (Op<P>::LS + kChunk - 1) / kChunk), 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(),
                                                t.L.get());
     }
   }
@@ -179,7 +238,7 @@ void l2p_final_order(Tree& t, int mode, int mask, double* F, hipStream_t st) {
   (void)mode;
   const int n = (int)t.n;
   PhaseTimer pt(t, kTL2P, st);
-  k_l2p_final<P><<<nblk(n, 128), 128, 0, st>>>(n, t.leaf_of.get(), t.perm.get(), t.cx.get(), t.cy.get(),
+  k_l2p_final<P><<<nblk(n, 64), 64, 0, st>>>(n, t.leaf_of.get(), t.perm.get(), t.cx.get(), t.cy.get(),
                                                t.cz.get(), t.px.get(), t.py.get(), t.pz.get(), t.L.get(),
                                                (mask & kPhaseP2P) ? t.Fp2p.get() : nullptr,
                                                (mask & kPhaseL2P) ? 1 : 0, F);
