@@ -159,8 +159,9 @@ struct Tree {
 
   // expansions, [ncells * Msize] and [ncells * Lsize] in FP64
   DevBuf<double> M, L;
-  // mixed-precision storage (scaled expansions), allocated on demand
+  // mixed precision: normalised FP32 expansions, FP64 copies of the top levels
   DevBuf<float> Mf, Lf;
+  DevBuf<double> Mtop, Ltop;
   DevBuf<double> Fp2p;            // per-target near-field result, sorted order
   Scratch scratch;
 

@@ -15,7 +15,8 @@
 //
 // A operand: lane l holds T[16 I + l%16][4 K + l/16] = D[idx(n + m)], read
 // from the group's D vector in LDS through a per-(step, lane) index table;
-// B operand: lane l holds M_src(l%16)[4 K + l/16]; accumulator register r of
+// B operand: lane l holds M_src(l%16)[4 K + l/16], read from the group's 16
+// source multipoles staged in LDS by coalesced row loads; accumulator register r of
 // lane l holds target column l%16 and row 16 I + acc_row(l/16, r), where
 // acc_row = 4 (l/16) + r for the f32 MFMA and (l/16) + 4 r for the f64 MFMA
 // (measured on gfx950 with single-element operands).
@@ -56,9 +57,18 @@ struct M2LPlan {
   static constexpr bool valid(int I, int K) {
     return 16 * I < LS && 4 * K < MS && slot_degree(16 * I) + slot_degree(4 * K + OFF) <= P;
   }
+  static constexpr int count_steps() {
+    int s = 0;
+    for (int K = 0; K < NK; ++K)
+      for (int I = 0; I < NI; ++I)
+        if (valid(I, K)) s = s + 1;
+    return s;
+  }
+  static constexpr int NST0 = count_steps();
   struct Tab {
     int nI[NK];
     int base[NK + 1];
+    short sI[NST0 + 1], sK[NST0 + 1];  // row block and slice of step s (K-major order)
   };
   // valid row blocks of slice K are a prefix (rows are ordered by degree)
   static constexpr Tab make() {
@@ -70,6 +80,10 @@ struct M2LPlan {
         if (valid(I, K)) c = I + 1;
       t.nI[K] = c;
       t.base[K] = s;
+      for (int I = 0; I < c; ++I) {
+        t.sI[s + I] = (short)I;
+        t.sK[s + I] = (short)K;
+      }
       s += c;
     }
     t.base[NK] = s;
@@ -111,7 +125,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * per + k;
 }
 
-template <int P, int SO, typename T, bool SCALED>
+template <int P, int SO, typename T, bool SCALED, bool STAGE_B>
 __global__ __launch_bounds__(64) void k_m2l_gemm(int nchunks, int nblocks, const int* __restrict__ tile_cells,
                                                  const int* __restrict__ chunk_tile,
                                                  const int* __restrict__ chunk_g0,
@@ -123,47 +137,115 @@ __global__ __launch_bounds__(64) void k_m2l_gemm(int nchunks, int nblocks, const
                                                  T* __restrict__ L) {
   using Plan = M2LPlan<P, SO>;
   constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NK = Plan::NK, NST = Plan::NSTEP;
+  constexpr int MSP = (MS | 1);  // odd row stride of the staged multipoles: conflict-free B reads
+  constexpr int NJ = (MS + 63) / 64;  // row segments of 64 slots
+  // columns prefetched in registers during the previous group's MFMAs
+  // (about 128 registers); the rest is loaded when the group is staged
+  constexpr int kWords = (int)(sizeof(T) / 4);
+  constexpr int PC0 = (128 / (NJ * kWords)) >= kTile ? kTile : (128 / (NJ * kWords));
+  constexpr int PC = STAGE_B ? PC0 : 0;
   using V4 = typename Vec4<T>::type;
   __shared__ short sidx[NST * 64];
-  __shared__ T sD[MS];
+  __shared__ T sD[MS + 1];                              // slot MS stays zero (empty A entries)
+  __shared__ T sB[STAGE_B ? kTile * MSP + 4 : 1];      // tail slots stay zero
+  __shared__ signed char sdeg[MS];
   const int lane = threadIdx.x;
   const int chunk = xcd_tile(blockIdx.x, nblocks);
   if (chunk >= nchunks) return;  // whole (single-wave) block
   const int tile = chunk_tile[chunk];
-  for (int i = lane; i < NST * 64; i += 64) sidx[i] = idxtab[i];
+  for (int i = lane; i < NST * 64; i += 64) {
+    const short v = idxtab[i];
+    sidx[i] = v < 0 ? (short)MS : v;
+  }
+  if (lane == 0) sD[MS] = T(0);
+  if constexpr (STAGE_B)
+    for (int i = lane; i < kTile * MSP + 4; i += 64) sB[i] = T(0);
+  for (int j = lane; j < MS; j += 64) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
   V4 acc[NI];
 #pragma unroll
   for (int I = 0; I < NI; ++I) acc[I] = V4{0, 0, 0, 0};
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
-  for (int g = g0; g < g1; ++g) {
+
+  T rB[PC > 0 ? PC : 1][NJ], rD[NJ];
+  auto prefetch = [&](int g) {
     const int key = group_key[g];
-    const int src = group_src[(size_t)g * kTile + col];
-    __syncthreads();  // previous group finished reading sD
-    for (int j = lane; j < MS; j += 64) sD[j] = Dtab[(size_t)key * MS + j];
-    T b[NK];
-    const int dl = SCALED ? key_dlevel[key] : 0;
-    static_for<NK>([&](auto K) {
-      constexpr int k0 = 4 * K;
-      const int m = k0 + kk;
-      T v = T(0);
-      if (src >= 0 && m < MS) v = M[(size_t)src * MS + m];
-      if constexpr (SCALED) {
-        constexpr int d0 = slot_degree(k0 + Plan::OFF), d1 = slot_degree(k0 + 1 + Plan::OFF);
-        constexpr int d2 = slot_degree(k0 + 2 + Plan::OFF), d3 = slot_degree(k0 + 3 + Plan::OFF);
-        const int d = kk == 0 ? d0 : (kk == 1 ? d1 : (kk == 2 ? d2 : d3));
-        v = ldexpf(v, dl * d);
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const int j = 64 * i + lane;
+      rD[i] = j < MS ? Dtab[(size_t)key * MS + j] : T(0);
+    }
+#pragma unroll
+    for (int c = 0; c < PC; ++c) {
+      const int src = group_src[(size_t)g * kTile + c];
+#pragma unroll
+      for (int i = 0; i < NJ; ++i) {
+        const int j = 64 * i + lane;
+        rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : T(0);
       }
-      b[K] = v;
-    });
-    __syncthreads();  // sD visible to the wave
-    static_for<NK>([&](auto K) {
-      constexpr int nI = Plan::tab.nI[K];
-      static_for<nI>([&](auto I) {
-        constexpr int s = Plan::tab.base[K] + I;
-        const int ix = sidx[s * 64 + lane];
-        const T a = ix >= 0 ? sD[ix] : T(0);
-        acc[I] = mfma16x16x4(a, b[K], acc[I]);
+    }
+  };
+  if (g0 < g1) prefetch(g0);
+  for (int g = g0; g < g1; ++g) {
+    const int dl = SCALED ? key_dlevel[group_key[g]] : 0;
+    const int mysrc = group_src[(size_t)g * kTile + col];
+    const T* Mcol = M + (size_t)(mysrc >= 0 ? mysrc : 0) * MS;
+    __syncthreads();  // previous group finished reading sD / sB
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const int j = 64 * i + lane;
+      if (j < MS) sD[j] = rD[i];
+    }
+    if constexpr (STAGE_B) {
+      auto put = [&](int c, int i, T v) {
+        const int j = 64 * i + lane;
+        if (j < MS) {
+          if (SCALED && dl != 0) v = ldexpf(v, dl * sdeg[j]);
+          sB[c * MSP + j] = v;
+        }
+      };
+#pragma unroll
+      for (int c = 0; c < PC; ++c)
+#pragma unroll
+        for (int i = 0; i < NJ; ++i) put(c, i, rB[c][i]);
+      for (int c = PC; c < kTile; ++c) {  // columns beyond the register prefetch
+        const int src = group_src[(size_t)g * kTile + c];
+#pragma unroll
+        for (int i = 0; i < NJ; ++i) {
+          const int j = 64 * i + lane;
+          put(c, i, (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : T(0));
+        }
+      }
+    }
+    __syncthreads();  // staged operands visible to the wave
+    if (g + 1 < g1) prefetch(g + 1);  // in flight during the MFMAs below
+    // batches of kBatch steps: all operand reads of a batch, then its MFMAs
+    // (the reads of the next batch overlap the asynchronous MFMAs)
+    constexpr int kBatch = 16;
+    static_for<(NST + kBatch - 1) / kBatch>([&](auto bb) {
+      T av[kBatch], bv[kBatch];
+      static_for<kBatch>([&](auto u) {
+        constexpr int s = bb * kBatch + u;
+        if constexpr (s < NST) {
+          constexpr int K = Plan::tab.sK[s];
+          av[u] = sD[sidx[s * 64 + lane]];
+          if constexpr (STAGE_B) {
+            bv[u] = sB[col * MSP + 4 * K + kk];
+          } else {
+            // consecutive slices of one column share 128-byte lines in L1
+            const int m = 4 * K + kk;
+            T v = (mysrc >= 0 && m < MS) ? Mcol[m] : T(0);
+            if (SCALED && dl != 0) v = ldexpf(v, dl * sdeg[m < MS ? m : 0]);
+            bv[u] = v;
+          }
+        }
+      });
+      static_for<kBatch>([&](auto u) {
+        constexpr int s = bb * kBatch + u;
+        if constexpr (s < NST) {
+          constexpr int I = Plan::tab.sI[s];
+          acc[I] = mfma16x16x4(av[u], bv[u], acc[I]);
+        }
       });
     });
   }

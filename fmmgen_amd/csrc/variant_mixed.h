@@ -1,0 +1,313 @@
+// Mixed-precision far field of one operator variant.
+//
+// Expansions are stored normalised by the cell half-width r and in FP32:
+//     Mhat_n = M_n / r^|n|,            Lhat_n = L_n r^(|n|+1).
+// In these variables every operator is the generated body applied to
+// coordinates divided by a half-width, with power-of-two corrections:
+//   P2M  Mhat = r^-so P2M_S(x/r, S)                 x = centre - particle
+//   M2M  Mhat_p = M2M(x/r_p, Mhat_c * 2^-|j|)       x = parent - child
+//   M2L  Lhat_A = sum Dhat Mhat_B 2^((lA-lB)|m|)    (MFMA GEMM, m2l_gemm.h)
+//   L2L  Lhat_c = 2^-(|n|+1) L2L(x/r_p, Lhat_p)     x = child - parent
+//   L2P  phi = phihat / r, F = Fhat / r^2           (L2P(x/r, Lhat))
+// M2M and L2L of parents on the top three levels run in FP64 at order >= 8
+// (the terms of the binomial shifts grow there); the near and far parts of
+// each target are summed in FP64.
+#pragma once
+
+#include "m2l_gemm.h"
+#include "variant_decl.h"
+
+namespace fmm {
+namespace FMM_VARIANT_NS {
+
+constexpr int kTopLevels = 3;
+
+template <int N>
+struct DegTab {
+  signed char d[N];
+  constexpr DegTab() : d{} {
+    for (int j = 0; j < N; ++j) d[j] = (signed char)slot_degree(j);
+  }
+};
+
+// Reads of a normalised child multipole, rescaled to the parent half-width.
+template <typename S, int OFF>
+struct HalfScaledIn {
+  const S* p;
+  __device__ __forceinline__ float operator[](int j) const {
+    constexpr DegTab<512> tab{};
+    return ldexpf((float)p[j], -tab.d[j + OFF]);
+  }
+};
+template <int OFF>
+struct HalfScaledIn64 {
+  const double* p;
+  __device__ __forceinline__ double operator[](int j) const {
+    constexpr DegTab<512> tab{};
+    return ldexp(p[j], -tab.d[j + OFF]);
+  }
+};
+
+// FP32 P2M, thread per (leaf, chunk); leaves on the top levels also keep an
+// FP64 copy for the FP64 top shifts.
+template <int P>
+__global__ __launch_bounds__(64) void k_p2m_mixed(int nleaves, const int* __restrict__ leaves,
+                                                  const int* __restrict__ pstart, const int* __restrict__ pcount,
+                                                  const double* __restrict__ cx, const double* __restrict__ cy,
+                                                  const double* __restrict__ cz, const double* __restrict__ cr,
+                                                  const double* __restrict__ px, const double* __restrict__ py,
+                                                  const double* __restrict__ pz, const double* __restrict__ pS,
+                                                  float* __restrict__ Mf, double* __restrict__ Mtop, int ntop) {
+  constexpr int MS = Op<P>::MS, NCH = (MS + kChunk - 1) / kChunk;
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= nleaves) return;
+  const int c = leaves[li];
+  const double x0 = cx[c], y0 = cy[c], z0 = cz[c], ri = 1.0 / cr[c];
+  double sscale = 1.0;
+#pragma unroll
+  for (int k = 0; k < SO; ++k) sscale *= ri;
+  const int b = pstart[c], e = b + pcount[c];
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < MS) ? LO + kChunk : MS;
+    ChunkAcc<float, LO, HI> acc;
+#pragma unroll
+    for (int k = 0; k < HI - LO; ++k) acc.v[k] = 0.f;
+    for (int j = b; j < e; ++j) {
+      float s[SS];
+#pragma unroll
+      for (int k = 0; k < SS; ++k) s[k] = (float)(pS[(size_t)j * SS + k] * sscale);
+      fmmgen_ops::P2M_S_o<P, float>((float)((x0 - px[j]) * ri), (float)((y0 - py[j]) * ri),
+                                    (float)((z0 - pz[j]) * ri), s, acc);
+    }
+#pragma unroll
+    for (int k = 0; k < HI - LO; ++k) {
+      Mf[(size_t)c * MS + LO + k] = acc.v[k];
+      if (c < ntop) Mtop[(size_t)c * MS + LO + k] = acc.v[k];
+    }
+  });
+}
+
+// FP32 M2M of the parents of one level (below the FP64 top levels).
+template <int P>
+__global__ __launch_bounds__(64) void k_m2m_mixed(int beg, int end, const int* __restrict__ first_child,
+                                                  const int* __restrict__ nchild, const double* __restrict__ cx,
+                                                  const double* __restrict__ cy, const double* __restrict__ cz,
+                                                  const double* __restrict__ cr, float* __restrict__ Mf) {
+  constexpr int MS = Op<P>::MS, NCH = (MS + kChunk - 1) / kChunk, OFF = nterms(SO - 1);
+  const int p = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= end) return;
+  const int k = nchild[p];
+  if (k == 0) return;
+  const int fc = first_child[p];
+  const double ri = 1.0 / cr[p];
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < MS) ? LO + kChunk : MS;
+    ChunkAcc<float, LO, HI> acc;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.f;
+    for (int c = fc; c < fc + k; ++c)
+      fmmgen_ops::M2M_o<P, float>((float)((cx[p] - cx[c]) * ri), (float)((cy[p] - cy[c]) * ri),
+                                  (float)((cz[p] - cz[c]) * ri),
+                                  HalfScaledIn<float, OFF>{Mf + (size_t)c * MS}, acc);
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) Mf[(size_t)p * MS + LO + j] = acc.v[j];
+  });
+}
+
+// FP64 M2M of a top-level parent; children below the top read the FP32
+// array.  The result is kept in FP64 and rounded into the FP32 array for M2L.
+template <int P>
+__global__ __launch_bounds__(64) void k_m2m_top(int beg, int end, const int* __restrict__ first_child,
+                                                const int* __restrict__ nchild, const double* __restrict__ cx,
+                                                const double* __restrict__ cy, const double* __restrict__ cz,
+                                                const double* __restrict__ cr, float* __restrict__ Mf,
+                                                double* __restrict__ Mtop, int ntop) {
+  constexpr int MS = Op<P>::MS, NCH = (MS + kChunk - 1) / kChunk, OFF = nterms(SO - 1);
+  const int p = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= end) return;
+  const int k = nchild[p];
+  if (k == 0) return;
+  const int fc = first_child[p];
+  const double ri = 1.0 / cr[p];
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < MS) ? LO + kChunk : MS;
+    ChunkAcc<double, LO, HI> acc;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.0;
+    for (int c = fc; c < fc + k; ++c) {
+      const double x = (cx[p] - cx[c]) * ri, y = (cy[p] - cy[c]) * ri, z = (cz[p] - cz[c]) * ri;
+      if (c < ntop)
+        fmmgen_ops::M2M_o<P, double>(x, y, z, HalfScaledIn64<OFF>{Mtop + (size_t)c * MS}, acc);
+      else
+        fmmgen_ops::M2M_o<P, double>(x, y, z, HalfScaledIn<float, OFF>{Mf + (size_t)c * MS}, acc);
+    }
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) {
+      Mtop[(size_t)p * MS + LO + j] = acc.v[j];
+      Mf[(size_t)p * MS + LO + j] = (float)acc.v[j];
+    }
+  });
+}
+
+// FP32 L2L of the children of one level (parents below the top levels).
+template <int P>
+__global__ __launch_bounds__(64) void k_l2l_mixed(int beg, int end, const int* __restrict__ parent,
+                                                  const double* __restrict__ cx, const double* __restrict__ cy,
+                                                  const double* __restrict__ cz, const double* __restrict__ cr,
+                                                  float* __restrict__ Lf) {
+  constexpr int LS = Op<P>::LS, NCH = (LS + kChunk - 1) / kChunk;
+  const int c = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= end) return;
+  const int p = parent[c];
+  const double ri = 1.0 / cr[p];
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < LS) ? LO + kChunk : LS;
+    ChunkAcc<float, LO, HI> acc;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.f;
+    fmmgen_ops::L2L_o<P, float>((float)((cx[c] - cx[p]) * ri), (float)((cy[c] - cy[p]) * ri),
+                                (float)((cz[c] - cz[p]) * ri), static_cast<const float*>(Lf + (size_t)p * LS), acc);
+    constexpr DegTab<512> tab{};
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) Lf[(size_t)c * LS + LO + j] += ldexpf(acc.v[j], -(tab.d[LO + j] + 1));
+  });
+}
+
+// FP64 L2L from a top-level parent.  A top-level child accumulates in the
+// FP64 array (seeded with its FP32 M2L result); a child below the top adds
+// into the FP32 array.
+template <int P>
+__global__ __launch_bounds__(64) void k_l2l_top(int beg, int end, const int* __restrict__ parent,
+                                                const double* __restrict__ cx, const double* __restrict__ cy,
+                                                const double* __restrict__ cz, const double* __restrict__ cr,
+                                                float* __restrict__ Lf, double* __restrict__ Ltop, int ntop) {
+  constexpr int LS = Op<P>::LS, NCH = (LS + kChunk - 1) / kChunk;
+  const int c = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= end) return;
+  const int p = parent[c];
+  const double ri = 1.0 / cr[p];
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < LS) ? LO + kChunk : LS;
+    ChunkAcc<double, LO, HI> acc;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.0;
+    fmmgen_ops::L2L_o<P, double>((cx[c] - cx[p]) * ri, (cy[c] - cy[p]) * ri, (cz[c] - cz[p]) * ri,
+                                 static_cast<const double*>(Ltop + (size_t)p * LS), acc);
+    constexpr DegTab<512> tab{};
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) {
+      const double v = ldexp(acc.v[j], -(tab.d[LO + j] + 1));
+      if (c < ntop) {
+        Ltop[(size_t)c * LS + LO + j] = (double)Lf[(size_t)c * LS + LO + j] + v;
+        Lf[(size_t)c * LS + LO + j] = (float)Ltop[(size_t)c * LS + LO + j];
+      } else {
+        Lf[(size_t)c * LS + LO + j] += (float)v;
+      }
+    }
+  });
+}
+
+static __global__ void k_seed_root(int LS, const float* __restrict__ Lf, double* __restrict__ Ltop) {
+  for (int j = threadIdx.x; j < LS; j += blockDim.x) Ltop[j] = (double)Lf[j];
+}
+
+// FP32 L2P on the normalised local expansion, plus the FP64 near field.
+template <int P>
+__global__ __launch_bounds__(64) void k_l2p_mixed(int n, const int* __restrict__ leaf_of, const int* __restrict__ perm,
+                                                  const double* __restrict__ cx, const double* __restrict__ cy,
+                                                  const double* __restrict__ cz, const double* __restrict__ cr,
+                                                  const double* __restrict__ px, const double* __restrict__ py,
+                                                  const double* __restrict__ pz, const float* __restrict__ Lf,
+                                                  const double* __restrict__ Fnear, int use_l2p,
+                                                  double* __restrict__ F) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double out[OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) out[k] = Fnear ? Fnear[(size_t)j * OUT + k] : 0.0;
+  if (use_l2p) {
+    const int c = leaf_of[j];
+    const double ri = 1.0 / cr[c];
+    float acc[OUT];
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) acc[k] = 0.f;
+    fmmgen_ops::L2P_o<P, float>((float)((px[j] - cx[c]) * ri), (float)((py[j] - cy[c]) * ri),
+                                (float)((pz[j] - cz[c]) * ri), static_cast<const float*>(Lf + (size_t)c * Op<P>::LS),
+                                acc);
+    // potential scales with 1/r, field with 1/r^2
+    constexpr int F0 = OUT == 3 ? 0 : 1;
+    if (OUT != 3) out[0] += (double)acc[0] * ri;
+#pragma unroll
+    for (int k = F0; k < OUT; ++k) out[k] += (double)acc[k] * ri * ri;
+  }
+  const int i = perm[j];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) F[(size_t)i * OUT + k] = out[k];
+}
+
+template <int P>
+void m2l_mixed(Tree& t, int mode, hipStream_t st);
+
+template <int P>
+void far_field_mixed(Tree& t, int mode, int mask, hipStream_t st) {
+  const int nc = t.ncells;
+  constexpr int MS = Op<P>::MS, LS = Op<P>::LS;
+  const bool top64 = P >= 8;
+  const int ntop = top64 ? t.level_off[std::min(kTopLevels, t.depth)] : 0;
+  t.Mf.alloc((size_t)nc * MS);
+  t.Lf.alloc((size_t)nc * LS);
+  t.Mtop.alloc((size_t)std::max(ntop, 1) * MS);
+  t.Ltop.alloc((size_t)std::max(ntop, 1) * LS);
+  FMM_HIP(hipMemsetAsync(t.Mf.get(), 0, sizeof(float) * nc * MS, st));
+  FMM_HIP(hipMemsetAsync(t.Lf.get(), 0, sizeof(float) * nc * LS, st));
+  if (mask & kPhaseP2M) {
+    PhaseTimer pt(t, kTP2M, st);
+    k_p2m_mixed<P><<<dim3(nblk(t.nleaves, 64), (MS + kChunk - 1) / kChunk), 64, 0, st>>>(
+        t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
+        t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.Mf.get(), t.Mtop.get(), ntop);
+  }
+  if (mask & kPhaseM2M) {
+    PhaseTimer pt(t, kTM2M, st);
+    for (int l = t.depth - 2; l >= 0; --l) {
+      const int b = t.level_off[l], e = t.level_off[l + 1];
+      const dim3 g(nblk(e - b, 64), (MS + kChunk - 1) / kChunk);
+      if (top64 && l < kTopLevels)
+        k_m2m_top<P><<<g, 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(),
+                                       t.cz.get(), t.cr.get(), t.Mf.get(), t.Mtop.get(), ntop);
+      else
+        k_m2m_mixed<P><<<g, 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(),
+                                         t.cz.get(), t.cr.get(), t.Mf.get());
+    }
+  }
+  if (mask & kPhaseM2L) {
+    PhaseTimer pt(t, kTM2L, st);
+    m2l_mixed<P>(t, mode, st);
+  }
+  if (mask & kPhaseL2L) {
+    PhaseTimer pt(t, kTL2L, st);
+    if (top64) k_seed_root<<<1, 64, 0, st>>>(LS, t.Lf.get(), t.Ltop.get());
+    for (int l = 1; l < t.depth; ++l) {
+      const int b = t.level_off[l], e = t.level_off[l + 1];
+      const dim3 g(nblk(e - b, 64), (LS + kChunk - 1) / kChunk);
+      if (top64 && l - 1 < kTopLevels)
+        k_l2l_top<P><<<g, 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
+                                       t.Lf.get(), t.Ltop.get(), ntop);
+      else
+        k_l2l_mixed<P><<<g, 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
+                                         t.Lf.get());
+    }
+  }
+}
+
+template <int P>
+void l2p_mixed(Tree& t, int mask, double* F, hipStream_t st) {
+  const int n = (int)t.n;
+  PhaseTimer pt(t, kTL2P, st);
+  k_l2p_mixed<P><<<nblk(n, 64), 64, 0, st>>>(n, t.leaf_of.get(), t.perm.get(), t.cx.get(), t.cy.get(), t.cz.get(),
+                                             t.cr.get(), t.px.get(), t.py.get(), t.pz.get(), t.Lf.get(),
+                                             (mask & kPhaseP2P) ? t.Fp2p.get() : nullptr, (mask & kPhaseL2P) ? 1 : 0,
+                                             F);
+}
+
+}  // namespace FMM_VARIANT_NS
+}  // namespace fmm

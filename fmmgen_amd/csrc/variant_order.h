@@ -116,6 +116,16 @@ __global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const doubl
   for (int j = 0; j < MS; ++j) Dtab[(size_t)k * MS + j] = (T)D[j];
 }
 
+// Staging of the source multipoles through LDS (coalesced rows) or direct
+// L1-cached loads of the B fragments; FMM_M2L_STAGE_B=0/1 selects.
+inline bool m2l_stage_b() {
+  static const int v = [] {
+    const char* e = getenv("FMM_M2L_STAGE_B");
+    return e ? atoi(e) : 1;
+  }();
+  return v != 0;
+}
+
 template <int P>
 void ensure_index_table(Tree& t, hipStream_t st) {
   if (t.sched.idxtab_order == P) return;
@@ -134,10 +144,13 @@ void m2l_plain(Tree& t, hipStream_t st) {
   S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
   const int grid = ((S.nchunks + 7) / 8) * 8;
-  k_m2l_gemm<P, SO, double, false><<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(),
-                                                        S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
-                                                        S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(),
-                                                        S.idxtab.get(), t.M.get(), t.L.get());
+  auto launch = [&](auto kern) {
+    kern<<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
+                              S.chunk_g1.get(), S.group_key.get(), S.group_src.get(), S.key_dlevel.get(),
+                              S.Dtab64.get(), S.idxtab.get(), t.M.get(), t.L.get());
+  };
+  if (m2l_stage_b()) launch(k_m2l_gemm<P, SO, double, false, true>);
+  else launch(k_m2l_gemm<P, SO, double, false, false>);
 }
 
 // L2L (example/calculate.cpp:219-234), x = child - parent, one thread per
@@ -201,9 +214,38 @@ __global__ void k_op_batch(int n, const double* __restrict__ xyz, const double* 
   if constexpr (OPID == kOpP2MS) fmmgen_ops::P2M_S_o<P, double>(x, y, z, a, o);
 }
 
+}  // namespace FMM_VARIANT_NS
+}  // namespace fmm
+
+#include "variant_mixed.h"
+
+namespace fmm {
+namespace FMM_VARIANT_NS {
+
+template <int P>
+void m2l_mixed(Tree& t, int mode, hipStream_t st) {
+  M2LSchedule& S = t.sched;
+  if (S.ngroups == 0) return;
+  (void)mode;
+  ensure_index_table<P>(t, st);
+  S.Dtab32.alloc((size_t)S.nkeys * Op<P>::MS);
+  k_dtab<P, float, true><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab32.get());
+  const int grid = ((S.nchunks + 7) / 8) * 8;
+  auto launch = [&](auto kern) {
+    kern<<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
+                              S.chunk_g1.get(), S.group_key.get(), S.group_src.get(), S.key_dlevel.get(),
+                              S.Dtab32.get(), S.idxtab.get(), t.Mf.get(), t.Lf.get());
+  };
+  if (m2l_stage_b()) launch(k_m2l_gemm<P, SO, float, true, true>);
+  else launch(k_m2l_gemm<P, SO, float, true, false>);
+}
+
 template <int P>
 void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
-  (void)mode;
+  if (mode != 0) {
+    far_field_mixed<P>(t, mode, mask, st);
+    return;
+  }
   const int nc = t.ncells;
   if (mask & kPhaseP2M) {
     PhaseTimer pt(t, kTP2M, st);
@@ -235,7 +277,10 @@ void far_field_order(Tree& t, int mode, int mask, hipStream_t st) {
 
 template <int P>
 void l2p_final_order(Tree& t, int mode, int mask, double* F, hipStream_t st) {
-  (void)mode;
+  if (mode != 0) {
+    l2p_mixed<P>(t, mask, F, st);
+    return;
+  }
   const int n = (int)t.n;
   PhaseTimer pt(t, kTL2P, st);
   k_l2p_final<P><<<nblk(n, 64), 64, 0, st>>>(n, t.leaf_of.get(), t.perm.get(), t.cx.get(), t.cy.get(),
