@@ -113,8 +113,8 @@ struct M2LSchedule {
   DevBuf<double> Dtab64;
   DevBuf<float> Dtab32;
   // (step, lane) -> D slot table of the GEMM plan, uploaded once per order
-  DevBuf<short> idxtab;
-  int idxtab_order = -1;
+  DevBuf<short> idxtab, idxtab32;
+  int idxtab_order = -1, idxtab32_order = -1;
   // work items: chunks of at most kGroupsPerChunk groups of one tile
   int nchunks = 0;
   DevBuf<int> chunk_tile, chunk_g0, chunk_g1;

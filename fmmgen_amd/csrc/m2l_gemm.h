@@ -261,6 +261,218 @@ __global__ __launch_bounds__(64) void k_m2l_gemm(int nchunks, int nblocks, const
   });
 }
 
+// ---------------------------------------------------------------- BF16x3
+// Split-precision M2L: each FP32 operand x is carried as hi = bf16(x) plus
+// lo = bf16(x - hi), and the product as hi*hi + hi*lo + lo*hi on the BF16
+// matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Operand
+// layout (measured on gfx950): A[l%16][8 (l/16) + i], B[8 (l/16) + j][l%16],
+// i, j = 0..7; the accumulator is laid out as for the f32 16x16 MFMA.
+template <int P, int SO>
+struct M2LPlan32 {
+  static constexpr int LS = lsize(P, SO), MS = msize(P, SO);
+  static constexpr int NI = (LS + 15) / 16, NK = (MS + 31) / 32;
+  static constexpr int OFF = nterms(SO - 1);
+  static constexpr bool valid(int I, int K) {
+    return 16 * I < LS && 32 * K < MS && slot_degree(16 * I) + slot_degree(32 * K + OFF) <= P;
+  }
+  static constexpr int count_steps() {
+    int s = 0;
+    for (int K = 0; K < NK; ++K)
+      for (int I = 0; I < NI; ++I)
+        if (valid(I, K)) s = s + 1;
+    return s;
+  }
+  static constexpr int NST0 = count_steps();
+  struct Tab {
+    int nI[NK];
+    int base[NK + 1];
+  };
+  static constexpr Tab make() {
+    Tab t{};
+    int s = 0;
+    for (int K = 0; K < NK; ++K) {
+      int c = 0;
+      for (int I = 0; I < NI; ++I)
+        if (valid(I, K)) c = I + 1;
+      t.nI[K] = c;
+      t.base[K] = s;
+      s += c;
+    }
+    t.base[NK] = s;
+    return t;
+  }
+  static constexpr Tab tab = make();
+  static constexpr int NSTEP = tab.base[NK];
+};
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// One FP32 value as packed (hi | lo << 16) bf16 halves.
+__device__ __forceinline__ unsigned pack_split(float x) {
+  const __bf16 h = (__bf16)x;
+  const __bf16 l = (__bf16)(x - (float)h);
+  return (unsigned)__builtin_bit_cast(unsigned short, h) | ((unsigned)__builtin_bit_cast(unsigned short, l) << 16);
+}
+
+// Eight packed values -> hi and lo operand vectors (two byte permutes per pair).
+__device__ __forceinline__ void unpack8(const unsigned (&w)[8], bf16x8& hi, bf16x8& lo) {
+  u32x4 h, l;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    h[i] = __builtin_amdgcn_perm(w[2 * i + 1], w[2 * i], 0x05040100u);
+    l[i] = __builtin_amdgcn_perm(w[2 * i + 1], w[2 * i], 0x07060302u);
+  }
+  hi = __builtin_bit_cast(bf16x8, h);
+  lo = __builtin_bit_cast(bf16x8, l);
+}
+
+__device__ __forceinline__ void split_bf16(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const __bf16 h = (__bf16)v[i];
+    hi[i] = h;
+    lo[i] = (__bf16)(v[i] - (float)h);
+  }
+}
+
+template <int P, int SO>
+__global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, const int* __restrict__ tile_cells,
+                                                   const int* __restrict__ chunk_tile,
+                                                   const int* __restrict__ chunk_g0,
+                                                   const int* __restrict__ chunk_g1,
+                                                   const int* __restrict__ group_key,
+                                                   const int* __restrict__ group_src,
+                                                   const int* __restrict__ key_dlevel,
+                                                   const float* __restrict__ Dtab, const short* __restrict__ idxtab,
+                                                   const float* __restrict__ M, float* __restrict__ L) {
+  using Plan = M2LPlan32<P, SO>;
+  constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NK = Plan::NK, NST = Plan::NSTEP;
+  constexpr int RS = ((32 * NK + 3) / 4) * 4 + 4;  // row stride: 16-byte aligned reads, zero tail
+  constexpr int NJ = (MS + 63) / 64;
+  using V4 = typename Vec4<float>::type;
+  __shared__ short sidx[NST * 8 * 64];
+  __shared__ unsigned sD[MS + 1];  // D split once per group: packed (hi | lo << 16)
+  __shared__ float sB[kTile * RS];
+  __shared__ signed char sdeg[MS];
+  const int lane = threadIdx.x;
+  const int chunk = xcd_tile(blockIdx.x, nblocks);
+  if (chunk >= nchunks) return;
+  const int tile = chunk_tile[chunk];
+  for (int i = lane; i < NST * 8 * 64; i += 64) {
+    const short v = idxtab[i];
+    sidx[i] = v < 0 ? (short)MS : v;
+  }
+  if (lane == 0) sD[MS] = 0u;
+  for (int i = lane; i < kTile * RS; i += 64) sB[i] = 0.f;
+  for (int j = lane; j < MS; j += 64) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
+  V4 acc[NI];
+#pragma unroll
+  for (int I = 0; I < NI; ++I) acc[I] = V4{0, 0, 0, 0};
+  const int col = lane & 15, kk = lane >> 4;
+  const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
+  float rD[NJ], rB[kTile][NJ];
+  auto prefetch = [&](int g) {
+    const int key = group_key[g];
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const int j = 64 * i + lane;
+      rD[i] = j < MS ? Dtab[(size_t)key * MS + j] : 0.f;
+    }
+#pragma unroll
+    for (int c = 0; c < kTile; ++c) {
+      const int src = group_src[(size_t)g * kTile + c];
+#pragma unroll
+      for (int i = 0; i < NJ; ++i) {
+        const int j = 64 * i + lane;
+        rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : 0.f;
+      }
+    }
+  };
+  if (g0 < g1) prefetch(g0);
+  for (int g = g0; g < g1; ++g) {
+    const int dl = key_dlevel[group_key[g]];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const int j = 64 * i + lane;
+      if (j < MS) sD[j] = pack_split(rD[i]);
+    }
+#pragma unroll
+    for (int c = 0; c < kTile; ++c)
+#pragma unroll
+      for (int i = 0; i < NJ; ++i) {
+        const int j = 64 * i + lane;
+        if (j < MS) sB[c * RS + j] = dl != 0 ? ldexpf(rB[c][i], dl * sdeg[j]) : rB[c][i];
+      }
+    __syncthreads();
+    if (g + 1 < g1) prefetch(g + 1);
+    static_for<NK>([&](auto K) {
+      constexpr int nI = Plan::tab.nI[K];
+      float bv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) bv[j] = sB[col * RS + 32 * K + 8 * kk + j];
+      bf16x8 bhi, blo;
+      split_bf16(bv, bhi, blo);
+      static_for<nI>([&](auto I) {
+        constexpr int s = Plan::tab.base[K] + I;
+        unsigned aw[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) aw[i] = sD[sidx[(s * 8 + i) * 64 + lane]];
+        bf16x8 ahi, alo;
+        unpack8(aw, ahi, alo);
+        acc[I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[I], 0, 0, 0);
+        acc[I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[I], 0, 0, 0);
+        acc[I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[I], 0, 0, 0);
+      });
+    });
+  }
+  const int cell = tile_cells[tile * kTile + col];
+  if (cell < 0) return;
+  float* Lc = L + (size_t)cell * LS;
+  static_for<NI>([&](auto I) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = 16 * I + acc_row<float>(kk, r);
+      if (n < LS) atomicAdd(&Lc[n], acc[I][r]);
+    }
+  });
+}
+
+// (step, element, lane) -> D slot table of the BF16 plan (-1: zero entry).
+template <int P, int SO>
+std::vector<short> m2l_index_table32() {
+  using Plan = M2LPlan32<P, SO>;
+  std::vector<int> mx, my, mz;
+  for (int d = 0; d <= 2 * P; ++d)
+    for (int x = d; x >= 0; --x)
+      for (int y = d - x; y >= 0; --y) {
+        mx.push_back(x);
+        my.push_back(y);
+        mz.push_back(d - x - y);
+      }
+  auto idx = [](int x, int y, int z) {
+    const int d = x + y + z;
+    const int a = d - x;
+    return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
+  };
+  std::vector<short> t((size_t)Plan::NSTEP * 8 * 64, -1);
+  for (int K = 0; K < Plan::NK; ++K)
+    for (int I = 0; I < Plan::tab.nI[K]; ++I) {
+      const int s = Plan::tab.base[K] + I;
+      for (int l = 0; l < 64; ++l)
+        for (int i = 0; i < 8; ++i) {
+          const int n = 16 * I + (l & 15), m = 32 * K + 8 * (l >> 4) + i;
+          if (n >= Plan::LS || m >= Plan::MS) continue;
+          const int mf = m + Plan::OFF;
+          if (mx[n] + my[n] + mz[n] + mx[mf] + my[mf] + mz[mf] > P) continue;
+          t[((size_t)s * 8 + i) * 64 + l] = (short)(idx(mx[n] + mx[mf], my[n] + my[mf], mz[n] + mz[mf]) - Plan::OFF);
+        }
+    }
+  return t;
+}
+
 // Host side: the (step, lane) -> D slot table of the plan (-1: zero entry).
 template <int P, int SO>
 std::vector<short> m2l_index_table() {

@@ -226,11 +226,24 @@ template <int P>
 void m2l_mixed(Tree& t, int mode, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (S.ngroups == 0) return;
-  (void)mode;
-  ensure_index_table<P>(t, st);
   S.Dtab32.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, float, true><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab32.get());
   const int grid = ((S.nchunks + 7) / 8) * 8;
+  if (mode == 2) {  // BF16x3 split precision
+    if (S.idxtab32_order != P) {
+      const std::vector<short> tab = m2l_index_table32<P, SO>();
+      S.idxtab32.alloc(tab.size());
+      FMM_HIP(hipMemcpyAsync(S.idxtab32.get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
+      FMM_HIP(hipStreamSynchronize(st));
+      S.idxtab32_order = P;
+    }
+    k_m2l_bf16x3<P, SO><<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(),
+                                             S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
+                                             S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(),
+                                             S.idxtab32.get(), t.Mf.get(), t.Lf.get());
+    return;
+  }
+  ensure_index_table<P>(t, st);
   auto launch = [&](auto kern) {
     kern<<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
                               S.chunk_g1.get(), S.group_key.get(), S.group_src.get(), S.key_dlevel.get(),

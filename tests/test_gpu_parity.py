@@ -165,6 +165,8 @@ MIXED_CASES = [
     ("uniform", 20000, 64, 8, 0.5, 0, "f32"),
     ("plummer", 20000, 32, 6, 0.4, 0, "f32"),
     ("plummer", 20000, 32, 6, 0.4, 1, "f32"),
+    ("uniform", 20000, 64, 8, 0.5, 0, "bf16x3"),
+    ("plummer", 20000, 32, 6, 0.4, 1, "bf16x3"),
 ]
 
 
