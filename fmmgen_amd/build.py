@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Compiled operator variants: generate_code() arguments of the device
 # header and the name of the variant table in the library.
 VARIANTS = {
-    "so0_pf": dict(order=9, source_order=0, potential=True, field=True),
+    "so0_pf": dict(order=13, source_order=0, potential=True, field=True),
     "so1_f": dict(order=9, source_order=1, potential=False, field=True),
 }
 
@@ -81,6 +81,12 @@ def build(jobs=None, verbose=False):
     units = generate()
     os.makedirs(OUT, exist_ok=True)
     srcs = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")] + units
+
+    def weight(path):   # longest compiles first: high-order units
+        stem = os.path.basename(path)[:-4]
+        tail = stem.rsplit("_o", 1)
+        return -int(tail[1]) if len(tail) == 2 and tail[1].isdigit() else 0
+    srcs.sort(key=weight)
     objs = []
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

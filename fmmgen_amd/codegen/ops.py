@@ -37,6 +37,7 @@ A program is a list of statements:
 Expressions are C expression strings in which literal rational constants
 appear as ``@num/den@`` and are printed per precision by the printers.
 """
+import re
 from fractions import Fraction
 from math import factorial
 
@@ -360,3 +361,36 @@ def gen_P2P(so, potential=True, field=True, harmonic=True):
         else:
             prog.out(j, f"-({_sum(terms)})", 2 * len(terms))
     return prog
+
+
+_NAME = re.compile(r"[A-Za-z_][A-Za-z0-9_]*")
+
+
+def prune(prog, lo, hi, name=None):
+    """Copy of ``prog`` that keeps only the outputs with slot in [lo, hi) and
+    the temporaries they need (backward liveness over the straight-line
+    program).  Used to emit one small body per output chunk."""
+    keep = []
+    live = set()
+    for st in reversed(prog.stmts):
+        kind = st[0]
+        if kind == "out":
+            if lo <= st[2] < hi:
+                keep.append(st)
+                live.update(_NAME.findall(st[3]))
+        elif kind == "tmp":
+            if st[1] in live:
+                keep.append(st)
+                live.update(_NAME.findall(st[2]))
+        elif kind in ("set", "put"):
+            if st[1] in live:
+                keep.append(st)
+                live.update(_NAME.findall(st[3]))
+        elif kind == "arr":
+            if st[1] in live:
+                keep.append(st)
+        else:
+            keep.append(st)
+    out = Program(name or f"{prog.name}_c{lo}", prog.args, prog.out_name)
+    out.stmts = list(reversed(keep))
+    return out

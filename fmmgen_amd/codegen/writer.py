@@ -104,23 +104,69 @@ def host_source(order, name, precision="double", potential=True, field=True,
     return header, source, ops_counts
 
 
+CHUNK = 32  # output slots per chunk body (matches the kernels' chunk size)
+CHUNKED = ("P2M_S", "M2M", "L2L")
+
+
 def device_source(order, potential=True, field=True, source_order=0,
-                  harmonic_derivs=False):
-    """Header-only HIP flavour: __device__ templates plus order wrappers."""
+                  harmonic_derivs=False, full_bodies_max_order=8):
+    """Header-only HIP flavour: __device__ templates plus order wrappers.
+
+    Besides the full operator bodies, P2M_S, M2M and L2L are also emitted as
+    one pruned body per chunk of CHUNK output slots (``<op>_<p>_c<k>``), so a
+    kernel that computes one chunk per thread compiles only what it uses.
+    Full bodies of orders above ``full_bodies_max_order`` are emitted only
+    for the small operators (L2P, Dgen, P2P and the chunked ones' pieces).
+    """
     start, progs = build_programs(order, source_order, potential, field,
                                   harmonic_derivs, gpu_extras=True)
     pr = HipPrinter()
-    text = _macros(start, order, source_order, potential, field) + pr.prelude()
+    text = _macros(start, order, source_order, potential, field)
+    text += f"#define FMMGEN_FULL_BODIES_MAXORDER {full_bodies_max_order}\n"
+    text += f"#define FMMGEN_CHUNK {CHUNK}\n" + pr.prelude()
     text += "namespace fmmgen_ops {\n"
     wrappers = {}
+    chunk_fns = {}
     for prog in progs:
-        text += pr.function(prog) + "\n"
         base, o = _base_name(prog.name)
+        big = base in ("M2M", "M2L", "L2L", "M2P", "P2M", "P2M_S")
+        if base in CHUNKED:
+            nout = max(st[2] for st in prog.stmts if st[0] == "out") + 1
+            names = []
+            for c0 in range(0, nout, CHUNK):
+                cp = ops.prune(prog, c0, c0 + CHUNK, name=f"{prog.name}_c{c0 // CHUNK}")
+                text += pr.function(cp) + "\n"
+                names.append(cp.name)
+            chunk_fns[(base, o)] = (prog, names)
+        if big and o is not None and o > full_bodies_max_order:
+            continue
+        text += pr.function(prog) + "\n"
         if o is not None:
             wrappers.setdefault(base, (prog, []))[1].append(o)
+    # chunk dispatch: <op>_chunk<P, C>(...)
+    for base in CHUNKED:
+        items = [(o, v) for (b, o), v in chunk_fns.items() if b == base]
+        if not items:
+            continue
+        prog0 = items[0][1][0]
+        fwd = [f"static_cast<A_{a}&&>({a})" if k != "scalar" else a for a, k in prog0.args]
+        tparams = ", ".join(["typename T"] + [f"typename A_{a}" for a, k in prog0.args if k != "scalar"])
+        params = ", ".join(f"T {a}" if k == "scalar" else f"A_{a}&& {a}" for a, k in prog0.args)
+        code = f"template <int P, int C, {tparams}>\n__device__ __forceinline__ void {base}_chunk({params}) {{\n"
+        first = True
+        for o, (prog, names) in sorted(items):
+            for ci, nm in enumerate(names):
+                kw = "if constexpr" if first else "else if constexpr"
+                code += f"  {kw} (P == {o} && C == {ci}) {nm}<T>({', '.join(fwd)});\n"
+                first = False
+        code += "}\n"
+        text += code
     for base, (prog, orders) in wrappers.items():
         args = [a for a, _ in prog.args]
         fwd = [f"static_cast<A_{a}&&>({a})" if k != "scalar" else a for a, k in prog.args]
+        big = base in ("M2M", "M2L", "L2L", "M2P", "P2M", "P2M_S")
+        if big:
+            orders = [o for o in orders if o <= full_bodies_max_order]
         sig = pr.signature(prog, name=base).rstrip(")") + ", int order)"
         code = sig + " {\n  switch (order) {\n"
         for o in orders:

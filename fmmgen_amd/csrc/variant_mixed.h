@@ -76,7 +76,7 @@ __global__ __launch_bounds__(64) void k_p2m_mixed(int nleaves, const int* __rest
       float s[SS];
 #pragma unroll
       for (int k = 0; k < SS; ++k) s[k] = (float)(pS[(size_t)j * SS + k] * sscale);
-      fmmgen_ops::P2M_S_o<P, float>((float)((x0 - px[j]) * ri), (float)((y0 - py[j]) * ri),
+      fmmgen_ops::P2M_S_chunk<P, LO / kChunk, float>((float)((x0 - px[j]) * ri), (float)((y0 - py[j]) * ri),
                                     (float)((z0 - pz[j]) * ri), s, acc);
     }
 #pragma unroll
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(64) void k_m2m_mixed(int beg, int end, const int* _
 #pragma unroll
     for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.f;
     for (int c = fc; c < fc + k; ++c)
-      fmmgen_ops::M2M_o<P, float>((float)((cx[p] - cx[c]) * ri), (float)((cy[p] - cy[c]) * ri),
+      fmmgen_ops::M2M_chunk<P, LO / kChunk, float>((float)((cx[p] - cx[c]) * ri), (float)((cy[p] - cy[c]) * ri),
                                   (float)((cz[p] - cz[c]) * ri),
                                   HalfScaledIn<float, OFF>{Mf + (size_t)c * MS}, acc);
 #pragma unroll
@@ -137,9 +137,9 @@ __global__ __launch_bounds__(64) void k_m2m_top(int beg, int end, const int* __r
     for (int c = fc; c < fc + k; ++c) {
       const double x = (cx[p] - cx[c]) * ri, y = (cy[p] - cy[c]) * ri, z = (cz[p] - cz[c]) * ri;
       if (c < ntop)
-        fmmgen_ops::M2M_o<P, double>(x, y, z, HalfScaledIn64<OFF>{Mtop + (size_t)c * MS}, acc);
+        fmmgen_ops::M2M_chunk<P, LO / kChunk, double>(x, y, z, HalfScaledIn64<OFF>{Mtop + (size_t)c * MS}, acc);
       else
-        fmmgen_ops::M2M_o<P, double>(x, y, z, HalfScaledIn<float, OFF>{Mf + (size_t)c * MS}, acc);
+        fmmgen_ops::M2M_chunk<P, LO / kChunk, double>(x, y, z, HalfScaledIn<float, OFF>{Mf + (size_t)c * MS}, acc);
     }
 #pragma unroll
     for (int j = 0; j < HI - LO; ++j) {
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(64) void k_l2l_mixed(int beg, int end, const int* _
     ChunkAcc<float, LO, HI> acc;
 #pragma unroll
     for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.f;
-    fmmgen_ops::L2L_o<P, float>((float)((cx[c] - cx[p]) * ri), (float)((cy[c] - cy[p]) * ri),
+    fmmgen_ops::L2L_chunk<P, LO / kChunk, float>((float)((cx[c] - cx[p]) * ri), (float)((cy[c] - cy[p]) * ri),
                                 (float)((cz[c] - cz[p]) * ri), static_cast<const float*>(Lf + (size_t)p * LS), acc);
     constexpr DegTab<512> tab{};
 #pragma unroll
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(64) void k_l2l_top(int beg, int end, const int* __r
     ChunkAcc<double, LO, HI> acc;
 #pragma unroll
     for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.0;
-    fmmgen_ops::L2L_o<P, double>((cx[c] - cx[p]) * ri, (cy[c] - cy[p]) * ri, (cz[c] - cz[p]) * ri,
+    fmmgen_ops::L2L_chunk<P, LO / kChunk, double>((cx[c] - cx[p]) * ri, (cy[c] - cy[p]) * ri, (cz[c] - cz[p]) * ri,
                                  static_cast<const double*>(Ltop + (size_t)p * LS), acc);
     constexpr DegTab<512> tab{};
 #pragma unroll

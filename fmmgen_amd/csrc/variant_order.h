@@ -31,7 +31,7 @@ struct ChunkAcc {
   __device__ __forceinline__ Ref operator[](int k) { return Ref{(k >= LO && k < HI) ? &v[k - LO] : nullptr}; }
 };
 
-constexpr int kChunk = 32;  // expansion slots per thread
+constexpr int kChunk = FMMGEN_CHUNK;  // expansion slots per thread (one pruned generated body per chunk)
 
 template <int NCH, typename F>
 __device__ __forceinline__ void dispatch_chunk(int ch, F&& f) {
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(64) void k_p2m(int nleaves, const int* __restrict__
 #pragma unroll
     for (int k = 0; k < HI - LO; ++k) acc.v[k] = 0.0;
     for (int j = b; j < e; ++j)
-      fmmgen_ops::P2M_S_o<P, double>(x0 - px[j], y0 - py[j], z0 - pz[j], pS + (size_t)j * SS, acc);
+      fmmgen_ops::P2M_S_chunk<P, LO / kChunk, double>(x0 - px[j], y0 - py[j], z0 - pz[j], pS + (size_t)j * SS, acc);
     double* Mc = M + (size_t)c * MS;
 #pragma unroll
     for (int k = 0; k < HI - LO; ++k) Mc[LO + k] = acc.v[k];
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(64) void k_m2m(int beg, int end, const int* __restr
 #pragma unroll
     for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.0;
     for (int c = fc; c < fc + k; ++c)
-      fmmgen_ops::M2M_o<P, double>(cx[p] - cx[c], cy[p] - cy[c], cz[p] - cz[c],
+      fmmgen_ops::M2M_chunk<P, LO / kChunk, double>(cx[p] - cx[c], cy[p] - cy[c], cz[p] - cz[c],
                                    static_cast<const double*>(M + (size_t)c * MS), acc);
     double* Mp = M + (size_t)p * MS;
 #pragma unroll
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(64) void k_l2l(int beg, int end, const int* __restr
     ChunkAcc<double, LO, HI> acc;
 #pragma unroll
     for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.0;
-    fmmgen_ops::L2L_o<P, double>(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p],
+    fmmgen_ops::L2L_chunk<P, LO / kChunk, double>(cx[c] - cx[p], cy[c] - cy[p], cz[c] - cz[p],
                                  static_cast<const double*>(L + (size_t)p * LS), acc);
     double* Lc = L + (size_t)c * LS;
 #pragma unroll
@@ -306,6 +306,12 @@ template <int P>
 void op_batch_order(int op, int n, const double* xyz, const double* in, int is, double* out, int os,
                     hipStream_t st) {
   const int g = nblk(n, 64);
+  if constexpr (P > FMMGEN_FULL_BODIES_MAXORDER) {
+    // whole-operator bodies are compiled up to FMMGEN_FULL_BODIES_MAXORDER
+    if (op != kOpL2P) throw Error{kInvalidArgument, "operator batch above the full-body order: only L2P"};
+    k_op_batch<P, kOpL2P><<<g, 64, 0, st>>>(n, xyz, in, is, out, os);
+    return;
+  }
   switch (op) {
     case kOpP2M: k_op_batch<P, kOpP2M><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;
     case kOpM2M: k_op_batch<P, kOpM2M><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;

@@ -18,6 +18,7 @@ from tests import util
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
+FULL_BODIES_MAXORDER = 8
 
 
 def _dev(a):
@@ -52,6 +53,13 @@ def test_operator_batches_match_oracle(gpu, so, outsz):
             "L2P": (xyz, rng.uniform(-1, 1, (nb, ls)), outsz, lib.oracle_op_L2P),
             "M2P": (xyz_far, rng.uniform(-1, 1, (nb, ms)), outsz, lib.oracle_op_M2P),
         }
+        if p > FULL_BODIES_MAXORDER:
+            # whole-operator batches above this order: L2P only (the
+            # chunked kernels are checked through the full FMM cases)
+            cases = {"L2P": cases["L2P"]}
+            with pytest.raises(_lib.FmmError):
+                T.op_batch("M2L", p, _dev(xyz), _dev(np.zeros((nb, ms))),
+                           torch.zeros(nb, ls, dtype=torch.float64, device="cuda"), so, outsz)
         for op, (x, inp, osz, fn) in cases.items():
             _log(f"op {op} so={so} p={p}")
             out = np.zeros((nb, osz))
@@ -125,6 +133,8 @@ FMM_CASES = [
     ("blobs", 30000, 64, 5, 0.5, 0),
     ("plummer", 20000, 32, 6, 0.4, 1),
     ("uniform", 10000, 16, 3, 0.5, 1),
+    ("uniform", 20000, 64, 10, 0.5, 0),
+    ("blobs", 20000, 64, 12, 0.5, 0),
 ]
 
 
@@ -167,6 +177,8 @@ MIXED_CASES = [
     ("plummer", 20000, 32, 6, 0.4, 1, "f32"),
     ("uniform", 20000, 64, 8, 0.5, 0, "bf16x3"),
     ("plummer", 20000, 32, 6, 0.4, 1, "bf16x3"),
+    ("blobs", 20000, 64, 12, 0.5, 0, "f32"),
+    ("uniform", 20000, 64, 10, 0.5, 0, "bf16x3"),
 ]
 
 
