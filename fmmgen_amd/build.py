@@ -69,15 +69,30 @@ def generate():
     return units
 
 
-def _compile(src, obj):
-    cmd = [HIPCC, *HIPFLAGS, "-c", src, "-o", obj]
+def _up_to_date(src, obj):
+    """True when obj and its dependency file are newer than every input."""
+    dep = obj[:-2] + ".d"
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return False
+    with open(dep) as f:
+        text = f.read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else []
+    t = os.path.getmtime(obj)
+    return all(os.path.exists(x) and os.path.getmtime(x) <= t for x in files + [src])
+
+
+def _compile(src, obj, incremental=False):
+    if incremental and _up_to_date(src, obj):
+        return obj
+    cmd = [HIPCC, *HIPFLAGS, "-MMD", "-MF", obj[:-2] + ".d", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n{r.stderr[-6000:]}")
     return obj
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, incremental=False):
+    """Compile every unit (incremental=True: only those whose inputs changed)."""
     units = generate()
     os.makedirs(OUT, exist_ok=True)
     srcs = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")] + units
@@ -94,7 +109,7 @@ def build(jobs=None, verbose=False):
         for s in srcs:
             obj = os.path.join(OUT, os.path.basename(s)[:-4] + ".o")
             objs.append(obj)
-            futs.append(ex.submit(_compile, s, obj))
+            futs.append(ex.submit(_compile, s, obj, incremental))
         for f in futs:
             f.result()
             if verbose:
@@ -111,5 +126,6 @@ def build(jobs=None, verbose=False):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--incremental", action="store_true")
     a = ap.parse_args()
-    print(build(a.j, verbose=True))
+    print(build(a.j, verbose=True, incremental=a.incremental))

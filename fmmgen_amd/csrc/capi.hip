@@ -193,6 +193,9 @@ int fmm_tree_info(fmm_tree_t tree, long long* info) {
     info[10] = t.sched.ntiles;
     info[11] = t.sched.ngroups;
     info[12] = t.sched.nkeys;
+    info[13] = t.n_p2p_items;
+    info[14] = 0;
+    info[15] = 0;
   });
 }
 

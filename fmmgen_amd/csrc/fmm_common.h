@@ -155,6 +155,16 @@ struct Tree {
   // leaf itself excluded), sorted by start: CSR over cells
   long long n_rng = 0;
   DevBuf<int> rng_off, rng_start, rng_len;
+  // P2P work items: the source sequence of a target leaf (own particles,
+  // then its merged ranges) cut into segments of at most kP2PSegment
+  // source particles, so that a leaf next to a dense cluster is spread
+  // over many waves.  Item = (leaf, first range (rb - 1 = own leaf),
+  // offset in that range, source count); split leaves accumulate with
+  // atomics into a zeroed output.
+  static constexpr int kP2PSegment = 8192;
+  long long n_p2p_items = 0;
+  int p2p_split = 0;  // some leaf has more than one item
+  DevBuf<int> p2p_item_leaf, p2p_item_r, p2p_item_o, p2p_item_n;
   M2LSchedule sched;
 
   // expansions, [ncells * Msize] and [ncells * Lsize] in FP64

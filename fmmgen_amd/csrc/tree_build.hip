@@ -420,7 +420,96 @@ __global__ void k_rng_fill(long long m, const unsigned long long* __restrict__ k
   atomicAdd(&rlen[r], pcount[srcB[i]]);
 }
 
+// Per leaf: number of P2P segments; per range: its length as 64-bit.
+__global__ void k_seg_count(int ncells, const int* __restrict__ nchild, const int* __restrict__ pcount,
+                            const int* __restrict__ rng_off, const long long* __restrict__ cum,
+                            long long* __restrict__ nseg, int seg) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= ncells) return;
+  long long k = 0;
+  if (nchild[A] == 0 && pcount[A] > 0) {
+    const long long tot = pcount[A] + cum[rng_off[A + 1]] - cum[rng_off[A]];
+    k = (tot + seg - 1) / seg;
+  }
+  nseg[A] = k;
+}
+
+__global__ void k_widen(long long m, const int* __restrict__ in, long long* __restrict__ out) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < m) out[i] = in[i];
+}
+
+// Items of one leaf: segment s starts at position s * seg of the source
+// sequence (own particles first, then the ranges in order).
+__global__ void k_seg_fill(int ncells, const int* __restrict__ nchild, const int* __restrict__ pcount,
+                           const int* __restrict__ rng_off, const long long* __restrict__ cum,
+                           const long long* __restrict__ nseg, const long long* __restrict__ seg_off, int seg,
+                           int* __restrict__ it_leaf, int* __restrict__ it_r, int* __restrict__ it_o,
+                           int* __restrict__ it_n) {
+  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+  if (A >= ncells || nseg[A] == 0) return;
+  const int nt = pcount[A], rb = rng_off[A], re = rng_off[A + 1];
+  const long long base = cum[rb], tot = nt + cum[re] - base;
+  for (long long s = 0; s < nseg[A]; ++s) {
+    const long long P = s * seg, w = seg_off[A] + s;
+    int r, o;
+    if (P < nt) {
+      r = rb - 1;
+      o = (int)P;
+    } else {
+      // last range r in [rb, re) with cum[r] - base <= P - nt
+      const long long q = P - nt + base;
+      int lo = rb, hi = re - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cum[mid] <= q) lo = mid;
+        else hi = mid - 1;
+      }
+      r = lo;
+      o = (int)(q - cum[r]);
+    }
+    it_leaf[w] = A;
+    it_r[w] = r;
+    it_o[w] = o;
+    it_n[w] = (int)((tot - P < seg) ? tot - P : seg);
+  }
+}
+
 }  // namespace
+
+// P2P work items (segments of each leaf's source sequence).
+void build_p2p_items(Tree& t, hipStream_t st) {
+  const int nc = t.ncells;
+  const long long nr = t.n_rng;
+  DevBuf<long long> len, cum, nseg, soff;
+  len.alloc(nr + 1);
+  cum.alloc(nr + 1);
+  FMM_HIP(hipMemsetAsync(len.get(), 0, sizeof(long long) * (nr + 1), st));
+  if (nr > 0) k_widen<<<blocks(nr, 256), 256, 0, st>>>(nr, t.rng_len.get(), len.get());
+  exclusive_scan_int(t.scratch, len.get(), cum.get(), nr + 1, st);
+  nseg.alloc(nc + 1);
+  soff.alloc(nc + 1);
+  FMM_HIP(hipMemsetAsync(nseg.get(), 0, sizeof(long long) * (nc + 1), st));
+  const int seg = Tree::kP2PSegment;
+  k_seg_count<<<blocks(nc, 256), 256, 0, st>>>(nc, t.nchild.get(), t.pcount.get(), t.rng_off.get(), cum.get(),
+                                               nseg.get(), seg);
+  exclusive_scan_int(t.scratch, nseg.get(), soff.get(), nc + 1, st);
+  long long total = 0;
+  FMM_HIP(hipMemcpyAsync(&total, soff.get() + nc, sizeof(long long), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  FMM_CHECK(total < (1LL << 31), kOutOfMemory, "P2P work items exceed 2^31");
+  t.n_p2p_items = total;
+  const long long na = std::max<long long>(total, 1);
+  t.p2p_item_leaf.alloc(na);
+  t.p2p_item_r.alloc(na);
+  t.p2p_item_o.alloc(na);
+  t.p2p_item_n.alloc(na);
+  k_seg_fill<<<blocks(nc, 256), 256, 0, st>>>(nc, t.nchild.get(), t.pcount.get(), t.rng_off.get(), cum.get(),
+                                              nseg.get(), soff.get(), seg, t.p2p_item_leaf.get(), t.p2p_item_r.get(),
+                                              t.p2p_item_o.get(), t.p2p_item_n.get());
+  FMM_HIP(hipStreamSynchronize(st));
+  t.p2p_split = total > t.nleaves;
+}
 
 // Merged source particle ranges of every P2P target (self excluded).
 void build_p2p_ranges(Tree& t, hipStream_t st) {
@@ -693,6 +782,7 @@ void build_lists_device(Tree& t, hipStream_t st) {
   build_csr(t, m2l, n_m2l, t.m2l_off, t.m2l_src, st);
   build_csr(t, p2p, n_p2p, t.p2p_off, t.p2p_src, st);
   build_p2p_ranges(t, st);
+  build_p2p_items(t, st);
   {
     DevBuf<unsigned long long> tot;
     tot.alloc(1);

@@ -16,32 +16,6 @@ __global__ void k_gather_S(int n, const int* __restrict__ perm, const double* __
   for (int k = 0; k < SS; ++k) pS[(size_t)j * SS + k] = S[(size_t)i * SS + k];
 }
 
-// P2P near field (example/calculate.cpp:24-40, 208-216): per target
-// particle over the particles of its P2P source leaves, self excluded.
-__global__ void k_p2p_simple(int n, const int* __restrict__ leaf_of, const int* __restrict__ off,
-                             const int* __restrict__ src, const int* __restrict__ pstart,
-                             const int* __restrict__ pcount, const double* __restrict__ px,
-                             const double* __restrict__ py, const double* __restrict__ pz,
-                             const double* __restrict__ pS, double* __restrict__ F) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const int A = leaf_of[j];
-  const double xj = px[j], yj = py[j], zj = pz[j];
-  double acc[OUT];
-#pragma unroll
-  for (int k = 0; k < OUT; ++k) acc[k] = 0.0;
-  for (int i = off[A]; i < off[A + 1]; ++i) {
-    const int B = src[i];
-    const int b = pstart[B], e = b + pcount[B];
-    for (int s = b; s < e; ++s) {
-      if (s == j) continue;
-      fmmgen_ops::P2P<double>(xj - px[s], yj - py[s], zj - pz[s], pS + (size_t)s * SS, acc);
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < OUT; ++k) F[(size_t)j * OUT + k] = acc[k];
-}
-
 // FP64 direct sum on sampled targets (example/calculate.cpp:252-264): one
 // block per target, sources strided over the lanes, tree reduction in LDS.
 __global__ void k_direct(int n, const double* __restrict__ pos, const double* __restrict__ S,
@@ -85,11 +59,14 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
                           out + (size_t)i * out_stride);
 }
 
-// P2P near field, one wavefront per target leaf.
+// P2P near field (example/calculate.cpp:24-40, 208-216), one wavefront per
+// work item: a segment of at most 8192 sources of one target leaf.
 //
 // The sources of a target leaf are its own particles (self pairs masked)
 // and the merged particle ranges of its other P2P leaves; they are staged
-// 64 at a time through LDS with coalesced loads.  A leaf of nt <= 32
+// 64 at a time through LDS with coalesced loads.  A leaf whose sources span
+// several items (a small leaf beside a dense cluster) adds its partial sums
+// atomically into a zeroed output; other leaves store.  A leaf of nt <= 32
 // particles is replicated over 64 / pow2(nt) lane slices that take
 // alternate sources of each chunk, and the slices are summed at the end.
 //   T = double: FP64 throughout (the reference arithmetic).
@@ -122,26 +99,31 @@ __device__ __forceinline__ void p2p_chunk(int filled, int sl, int nslice, int ts
 }
 
 template <typename T>
-__global__ __launch_bounds__(64) void k_p2p_leaf(int nleaves, int nblocks, const int* __restrict__ leaves,
-                                                 const int* __restrict__ pstart, const int* __restrict__ pcount,
-                                                 const int* __restrict__ rng_off, const int* __restrict__ rng_start,
-                                                 const int* __restrict__ rng_len, const double* __restrict__ px,
-                                                 const double* __restrict__ py, const double* __restrict__ pz,
-                                                 const double* __restrict__ pS, double* __restrict__ F) {
+__global__ __launch_bounds__(64) void k_p2p_leaf(int nitems, int nblocks, const int* __restrict__ it_leaf,
+                                                 const int* __restrict__ it_r, const int* __restrict__ it_o,
+                                                 const int* __restrict__ it_n, const int* __restrict__ pstart,
+                                                 const int* __restrict__ pcount, const int* __restrict__ rng_off,
+                                                 const int* __restrict__ rng_start, const int* __restrict__ rng_len,
+                                                 const double* __restrict__ px, const double* __restrict__ py,
+                                                 const double* __restrict__ pz, const double* __restrict__ pS,
+                                                 int split, double* __restrict__ F) {
   constexpr bool kKahan = sizeof(T) == 4;
   __shared__ double sx[64], sy[64], sz[64];
   __shared__ T sS[64 * SS];
   __shared__ int sid[64];
   const int b = blockIdx.x;
-  const int li = (b % 8) * ((nblocks + 7) / 8) + b / 8;  // XCD-contiguous leaf ranges
-  if (li >= nleaves) return;
-  const int A = leaves[li];
+  const int it = (b % 8) * ((nblocks + 7) / 8) + b / 8;  // XCD-contiguous item ranges
+  if (it >= nitems) return;
+  const int A = it_leaf[it];
   const int lane = threadIdx.x;
   const int t0 = pstart[A], nt = pcount[A];
   int tpad = 1;
   while (tpad < nt && tpad < 64) tpad <<= 1;
   const int nslice = 64 / tpad;
   const int rb = rng_off[A], re = rng_off[A + 1];
+  // a leaf cut into several items accumulates atomically
+  // (the items of a leaf are contiguous)
+  const bool atomic = split && ((it > 0 && it_leaf[it - 1] == A) || (it + 1 < nitems && it_leaf[it + 1] == A));
   for (int tb = 0; tb < nt; tb += 64) {
     const int tl = tb + (lane % tpad), sl = lane / tpad;
     const bool active = tl < nt;
@@ -151,16 +133,17 @@ __global__ __launch_bounds__(64) void k_p2p_leaf(int nleaves, int nblocks, const
 #pragma unroll
     for (int k = 0; k < OUT; ++k) sum[k] = comp[k] = T(0);
     // source sequence: the own leaf (virtual range rb - 1, self pairs
-    // masked), then the merged ranges rb .. re - 1
-    int r = rb - 1, o = 0;
-    while (r < re) {
+    // masked), then the merged ranges rb .. re - 1; this item covers
+    // it_n[it] sources from range it_r[it], offset it_o[it]
+    int r = it_r[it], o = it_o[it], left = it_n[it];
+    while (left > 0) {
       const bool has_self = r < rb;
       int filled = 0;
       __syncthreads();  // previous chunk consumed
-      while (filled < 64 && r < re) {
+      while (filled < 64 && left > 0 && r < re) {
         const int rs = (r < rb) ? t0 : rng_start[r];
         const int rl = (r < rb) ? nt : rng_len[r];
-        const int take = min(64 - filled, rl - o);
+        const int take = min(min(64 - filled, rl - o), left);
         if (lane >= filled && lane < filled + take) {
           const int j = rs + o + (lane - filled);
           sx[lane] = px[j];
@@ -172,11 +155,13 @@ __global__ __launch_bounds__(64) void k_p2p_leaf(int nleaves, int nblocks, const
         }
         filled += take;
         o += take;
+        left -= take;
         if (o == rl) {
           ++r;
           o = 0;
         }
       }
+      if (filled == 0) break;  // defensive: sequence exhausted
       __syncthreads();  // chunk visible
       T part[OUT];
 #pragma unroll
@@ -204,23 +189,36 @@ __global__ __launch_bounds__(64) void k_p2p_leaf(int nleaves, int nblocks, const
     for (int k = 0; k < OUT; ++k) {
       double v = kKahan ? (double)sum[k] - (double)comp[k] : (double)sum[k];
       for (int w = tpad; w < 64; w <<= 1) v += __shfl_xor(v, w, 64);
-      if (active && sl == 0) F[(size_t)tslot * OUT + k] = v;
+      if (active && sl == 0) {
+        if (atomic) atomicAdd(&F[(size_t)tslot * OUT + k], v);
+        else F[(size_t)tslot * OUT + k] = v;
+      }
     }
   }
 }
 
+__global__ void k_fill_zero(long long n, double* __restrict__ a) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
 void near_field(Tree& t, int mode, hipStream_t st) {
-  const int nl = t.nleaves;
+  const int ni = (int)t.n_p2p_items;
   PhaseTimer pt(t, kTP2P, st);
-  const int grid = ((nl + 7) / 8) * 8;
-  if (mode == 0)
-    k_p2p_leaf<double><<<grid, 64, 0, st>>>(nl, grid, t.leaves.get(), t.pstart.get(), t.pcount.get(),
-                                            t.rng_off.get(), t.rng_start.get(), t.rng_len.get(), t.px.get(),
-                                            t.py.get(), t.pz.get(), t.pS.get(), t.Fp2p.get());
-  else
-    k_p2p_leaf<float><<<grid, 64, 0, st>>>(nl, grid, t.leaves.get(), t.pstart.get(), t.pcount.get(),
-                                           t.rng_off.get(), t.rng_start.get(), t.rng_len.get(), t.px.get(),
-                                           t.py.get(), t.pz.get(), t.pS.get(), t.Fp2p.get());
+  if (t.p2p_split) {
+    const long long m = (long long)t.n * OUT;
+    k_fill_zero<<<(int)((m + 255) / 256), 256, 0, st>>>(m, t.Fp2p.get());
+  }
+  if (ni == 0) return;
+  const int grid = ((ni + 7) / 8) * 8;
+  auto launch = [&](auto kern) {
+    kern<<<grid, 64, 0, st>>>(ni, grid, t.p2p_item_leaf.get(), t.p2p_item_r.get(), t.p2p_item_o.get(),
+                              t.p2p_item_n.get(), t.pstart.get(), t.pcount.get(), t.rng_off.get(),
+                              t.rng_start.get(), t.rng_len.get(), t.px.get(), t.py.get(), t.pz.get(), t.pS.get(),
+                              t.p2p_split, t.Fp2p.get());
+  };
+  if (mode == 0) launch(k_p2p_leaf<double>);
+  else launch(k_p2p_leaf<float>);
 }
 
 template <int P>

@@ -65,10 +65,10 @@ class Tree:
 
     # ----------------------------------------------------------- properties
     def info(self):
-        a = (ctypes.c_longlong * 13)()
+        a = (ctypes.c_longlong * 16)()
         _lib.check(_lib.load().fmm_tree_info(self._h, a))
         keys = ("ncells", "nleaves", "levels", "n_m2l", "n_p2p", "n", "order", "source_order",
-                "p2p_particle_pairs", "ncrit", "m2l_tiles", "m2l_groups", "m2l_keys")
+                "p2p_particle_pairs", "ncrit", "m2l_tiles", "m2l_groups", "m2l_keys", "p2p_items")
         return dict(zip(keys, list(a)))
 
     def timing(self, enable):

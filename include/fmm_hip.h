@@ -80,9 +80,13 @@ int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncri
                    int source_order, int output_size, void* stream, fmm_tree_t* out);
 int fmm_tree_destroy(fmm_tree_t tree);
 
-/* info[0..12] = ncells, nleaves, levels, n_m2l, n_p2p, n, order,
+/* info must hold FMM_TREE_INFO_LEN entries:
+ * info[0..12] = ncells, nleaves, levels, n_m2l, n_p2p, n, order,
  * source_order, P2P particle pairs (self pairs excluded), ncrit,
- * M2L target tiles, M2L (tile, offset) groups, distinct M2L offsets */
+ * M2L target tiles, M2L (tile, offset) groups, distinct M2L offsets;
+ * info[13] = P2P work items (source segments of the target leaves);
+ * info[14..15] = 0 (reserved) */
+#define FMM_TREE_INFO_LEN 16
 int fmm_tree_info(fmm_tree_t tree, long long* info);
 
 /* Rebinds the borrowed source strengths (same geometry, new sources). */

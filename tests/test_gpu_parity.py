@@ -160,6 +160,32 @@ def test_fmm_plain_f64_matches_oracle(gpu, kind, n, ncrit, order, theta, so):
         assert util.rel_norm(F.cpu().numpy(), Fo2) < TOL, mask
 
 
+def test_fmm_split_p2p_leaves_match_oracle(gpu):
+    """A dense cluster beside sparse background: background leaves next to
+    the cluster have far more than one segment of sources, so their P2P
+    work is spread over several items that add atomically."""
+    from fmmgen_amd import build_tree
+    rng = np.random.default_rng(31)
+    n_c, n_b = 24000, 400
+    pos = np.vstack([0.5 + 0.004 * rng.standard_normal((n_c, 3)), rng.random((n_b, 3))])
+    n = len(pos)
+    q = rng.uniform(-1, 1, (n, 1))
+    _log(f"fmm split leaves n={n}")
+    ot = O.OracleTree(pos, q, 64, 5, 0.5)
+    Fo = ot.compute_field_fmm()
+    gt = build_tree(_dev(pos), _dev(q), n, 64, 5, 0.5)
+    info = gt.info()
+    assert info["p2p_items"] > info["nleaves"]
+    for mode, tol in (("plain", TOL), ("f32", 1e-6)):
+        F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+        gt.compute_field_fmm(F, mode=mode)
+        err = util.rel_norm(F.cpu().numpy()[:, 1:], Fo[:, 1:])
+        _log(f"  {mode} field difference {err:.3e}")
+        assert err < tol
+        gt.compute_field_fmm(F, mode=mode, phases=8)
+        assert util.rel_norm(F.cpu().numpy(), ot.compute_field_fmm(8)) < tol
+
+
 def test_direct_sampled_matches_oracle(gpu):
     from fmmgen_amd import direct_sum
     pos, q = util.uniform(20000, 21)
