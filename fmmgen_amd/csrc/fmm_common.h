@@ -162,6 +162,11 @@ struct Tree {
   // offset in that range, source count); split leaves accumulate with
   // atomics into a zeroed output.
   static constexpr int kP2PSegment = 8192;
+  // Per M2L group truncation order (learned order); empty: every group at
+  // the tree's order.
+  DevBuf<signed char> group_order;
+  bool use_group_order = false;
+  const signed char* group_order_ptr() const { return use_group_order ? group_order.get() : nullptr; }
   long long n_p2p_items = 0;
   int p2p_split = 0;  // some leaf has more than one item
   DevBuf<int> p2p_item_leaf, p2p_item_r, p2p_item_o, p2p_item_n;

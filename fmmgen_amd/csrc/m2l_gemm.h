@@ -3,8 +3,10 @@
 // For the pairs of one (tile, key) group the translation is the matrix
 // T[n][m] = D_{n+m}(x), x = target centre - source centre, non-zero only for
 // |n| + |m| <= p (fmmgen/expansions.py:144-172: L_n += sum_m D_{n+m} M_m).
-// One wavefront owns a tile of 16 target cells and computes
+// One block of W wavefronts owns a tile of 16 target cells and computes
 //     L[n][t] += sum_m T[n][m] * M_src(t)[m]        (t = 0..15)
+// sharing the staged operands in LDS; wave w issues the row blocks I = w
+// mod W.
 // with 16x16x4 MFMAs: A = 16 rows of T by 4 multipole slots, B = the same 4
 // slots of the 16 gathered source multipoles, the 16x16 accumulator block
 // holds 16 rows of the tile's local expansions.  Row blocks I and slot
@@ -125,139 +127,175 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * per + k;
 }
 
-template <int P, int SO, typename T, bool SCALED, bool STAGE_B>
-__global__ __launch_bounds__(64) void k_m2l_gemm(int nchunks, int nblocks, const int* __restrict__ tile_cells,
-                                                 const int* __restrict__ chunk_tile,
-                                                 const int* __restrict__ chunk_g0,
-                                                 const int* __restrict__ chunk_g1,
-                                                 const int* __restrict__ group_key,
-                                                 const int* __restrict__ group_src,
-                                                 const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
-                                                 const short* __restrict__ idxtab, const T* __restrict__ M,
-                                                 T* __restrict__ L) {
+// Steps of wave wv of a W-wave block (row blocks I = wv mod W), ordered by
+// the lowest total degree deg(row block) + deg(slice) so that a group
+// truncated at order q runs a prefix: cnt[q] steps.
+template <int P, int SO, int W, int WV>
+struct WavePlan {
   using Plan = M2LPlan<P, SO>;
-  constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NK = Plan::NK, NST = Plan::NSTEP;
-  constexpr int MSP = (MS | 1);  // odd row stride of the staged multipoles: conflict-free B reads
-  constexpr int NJ = (MS + 63) / 64;  // row segments of 64 slots
-  // columns prefetched in registers during the previous group's MFMAs
-  // (about 128 registers); the rest is loaded when the group is staged
-  constexpr int kWords = (int)(sizeof(T) / 4);
-  constexpr int PC0 = (128 / (NJ * kWords)) >= kTile ? kTile : (128 / (NJ * kWords));
-  constexpr int PC = STAGE_B ? PC0 : 0;
+  struct Tab {
+    short s[Plan::NSTEP + 1];
+    int n;
+    int cnt[P + 1];
+  };
+  static constexpr int lowdeg(int s) {
+    return slot_degree(16 * Plan::tab.sI[s]) + slot_degree(4 * Plan::tab.sK[s] + Plan::OFF);
+  }
+  static constexpr Tab make() {
+    Tab t{};
+    int n = 0;
+    for (int d = 0; d <= P; ++d) {
+      for (int s = 0; s < Plan::NSTEP; ++s)
+        if (Plan::tab.sI[s] % W == WV && lowdeg(s) == d) t.s[n++] = (short)s;
+      t.cnt[d] = n;
+    }
+    t.n = n;
+    return t;
+  }
+  static constexpr Tab tab = make();
+};
+
+// Waves per block: the LDS tables of a chunk are shared by W waves.
+template <int P, int SO, typename T>
+constexpr int m2l_waves() {
+  using Plan = M2LPlan<P, SO>;
+  constexpr int lds = Plan::NSTEP * 64 * 2 + (int)sizeof(T) * (kTile * (Plan::MS | 1) + Plan::MS);
+  return lds > 80 * 1024 ? 8 : 4;
+}
+
+template <int W, typename F>
+__device__ __forceinline__ void dispatch_wave(int w, F&& f) {
+  static_for<W>([&](auto WV) {
+    if (w == WV) f(WV);
+  });
+}
+
+// group_order (optional, per group): the order q <= P the group's pairs are
+// truncated at: D entries of degree > q are zeroed and the steps whose
+// lowest degree exceeds q are not issued.
+template <int P, int SO, typename T, bool SCALED, int W>
+__global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, const int* __restrict__ tile_cells,
+                                                     const int* __restrict__ chunk_tile,
+                                                     const int* __restrict__ chunk_g0,
+                                                     const int* __restrict__ chunk_g1,
+                                                     const int* __restrict__ group_key,
+                                                     const int* __restrict__ group_src,
+                                                     const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
+                                                     const short* __restrict__ idxtab,
+                                                     const signed char* __restrict__ group_order,
+                                                     const T* __restrict__ M, T* __restrict__ L) {
+  using Plan = M2LPlan<P, SO>;
+  constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NST = Plan::NSTEP;
+  constexpr int MSP = (MS | 1);          // odd row stride of the staged multipoles: conflict-free B reads
+  constexpr int NJ = (MS + 63) / 64;     // 64-slot segments of one column
+  constexpr int CPW = kTile / W;         // columns staged by each wave
+  constexpr int DJ = (NJ + W - 1) / W;   // D segments staged by each wave
   using V4 = typename Vec4<T>::type;
   __shared__ short sidx[NST * 64];
-  __shared__ T sD[MS + 1];                              // slot MS stays zero (empty A entries)
-  __shared__ T sB[STAGE_B ? kTile * MSP + 4 : 1];      // tail slots stay zero
+  __shared__ T sD[MS + 1];                // slot MS stays zero (empty A entries)
+  __shared__ T sB[kTile * MSP + 4];       // tail slots stay zero
   __shared__ signed char sdeg[MS];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int chunk = xcd_tile(blockIdx.x, nblocks);
-  if (chunk >= nchunks) return;  // whole (single-wave) block
+  if (chunk >= nchunks) return;  // uniform over the block
   const int tile = chunk_tile[chunk];
-  for (int i = lane; i < NST * 64; i += 64) {
+  for (int i = tid; i < NST * 64; i += 64 * W) {
     const short v = idxtab[i];
     sidx[i] = v < 0 ? (short)MS : v;
   }
-  if (lane == 0) sD[MS] = T(0);
-  if constexpr (STAGE_B)
-    for (int i = lane; i < kTile * MSP + 4; i += 64) sB[i] = T(0);
-  for (int j = lane; j < MS; j += 64) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
-  V4 acc[NI];
-#pragma unroll
-  for (int I = 0; I < NI; ++I) acc[I] = V4{0, 0, 0, 0};
+  if (tid == 0) sD[MS] = T(0);
+  for (int i = tid; i < kTile * MSP + 4; i += 64 * W) sB[i] = T(0);
+  for (int j = tid; j < MS; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
 
-  T rB[PC > 0 ? PC : 1][NJ], rD[NJ];
-  auto prefetch = [&](int g) {
-    const int key = group_key[g];
+  dispatch_wave<W>(w, [&](auto WV) {
+    constexpr int wv = WV;
+    using WP = WavePlan<P, SO, W, wv>;
+    constexpr int NIW = (NI - wv + W - 1) / W;  // row blocks wv, wv + W, ...
+    constexpr int NSW = WP::tab.n;
+    V4 acc[NIW > 0 ? NIW : 1];
 #pragma unroll
-    for (int i = 0; i < NJ; ++i) {
-      const int j = 64 * i + lane;
-      rD[i] = j < MS ? Dtab[(size_t)key * MS + j] : T(0);
-    }
+    for (int i = 0; i < (NIW > 0 ? NIW : 1); ++i) acc[i] = V4{0, 0, 0, 0};
+    T rD[DJ], rB[CPW][NJ];
+    auto prefetch = [&](int g) {
+      const int key = group_key[g];
 #pragma unroll
-    for (int c = 0; c < PC; ++c) {
-      const int src = group_src[(size_t)g * kTile + c];
-#pragma unroll
-      for (int i = 0; i < NJ; ++i) {
-        const int j = 64 * i + lane;
-        rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : T(0);
+      for (int i = 0; i < DJ; ++i) {
+        const int j = 64 * (wv + W * i) + lane;
+        rD[i] = j < MS ? Dtab[(size_t)key * MS + j] : T(0);
       }
-    }
-  };
-  if (g0 < g1) prefetch(g0);
-  for (int g = g0; g < g1; ++g) {
-    const int dl = SCALED ? key_dlevel[group_key[g]] : 0;
-    const int mysrc = group_src[(size_t)g * kTile + col];
-    const T* Mcol = M + (size_t)(mysrc >= 0 ? mysrc : 0) * MS;
-    __syncthreads();  // previous group finished reading sD / sB
 #pragma unroll
-    for (int i = 0; i < NJ; ++i) {
-      const int j = 64 * i + lane;
-      if (j < MS) sD[j] = rD[i];
-    }
-    if constexpr (STAGE_B) {
-      auto put = [&](int c, int i, T v) {
-        const int j = 64 * i + lane;
-        if (j < MS) {
-          if (SCALED && dl != 0) v = ldexpf(v, dl * sdeg[j]);
-          sB[c * MSP + j] = v;
-        }
-      };
-#pragma unroll
-      for (int c = 0; c < PC; ++c)
-#pragma unroll
-        for (int i = 0; i < NJ; ++i) put(c, i, rB[c][i]);
-      for (int c = PC; c < kTile; ++c) {  // columns beyond the register prefetch
-        const int src = group_src[(size_t)g * kTile + c];
+      for (int c = 0; c < CPW; ++c) {
+        const int src = group_src[(size_t)g * kTile + wv + W * c];
 #pragma unroll
         for (int i = 0; i < NJ; ++i) {
           const int j = 64 * i + lane;
-          put(c, i, (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : T(0));
+          rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : T(0);
         }
       }
-    }
-    __syncthreads();  // staged operands visible to the wave
-    if (g + 1 < g1) prefetch(g + 1);  // in flight during the MFMAs below
-    // batches of kBatch steps: all operand reads of a batch, then its MFMAs
-    // (the reads of the next batch overlap the asynchronous MFMAs)
-    constexpr int kBatch = 16;
-    static_for<(NST + kBatch - 1) / kBatch>([&](auto bb) {
-      T av[kBatch], bv[kBatch];
-      static_for<kBatch>([&](auto u) {
-        constexpr int s = bb * kBatch + u;
-        if constexpr (s < NST) {
-          constexpr int K = Plan::tab.sK[s];
-          av[u] = sD[sidx[s * 64 + lane]];
-          if constexpr (STAGE_B) {
-            bv[u] = sB[col * MSP + 4 * K + kk];
-          } else {
-            // consecutive slices of one column share 128-byte lines in L1
-            const int m = 4 * K + kk;
-            T v = (mysrc >= 0 && m < MS) ? Mcol[m] : T(0);
-            if (SCALED && dl != 0) v = ldexpf(v, dl * sdeg[m < MS ? m : 0]);
-            bv[u] = v;
+    };
+    if (g0 < g1) prefetch(g0);
+    for (int g = g0; g < g1; ++g) {
+      const int q = group_order ? (int)group_order[g] : P;
+      const int dl = SCALED ? key_dlevel[group_key[g]] : 0;
+      __syncthreads();  // previous group finished reading sD / sB
+#pragma unroll
+      for (int i = 0; i < DJ; ++i) {
+        const int j = 64 * (wv + W * i) + lane;
+        if (j < MS) sD[j] = sdeg[j] <= q ? rD[i] : T(0);
+      }
+#pragma unroll
+      for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int i = 0; i < NJ; ++i) {
+          const int j = 64 * i + lane;
+          if (j < MS) {
+            T v = rB[c][i];
+            if (SCALED && dl != 0) v = ldexpf(v, dl * sdeg[j]);
+            sB[(wv + W * c) * MSP + j] = v;
           }
         }
-      });
-      static_for<kBatch>([&](auto u) {
-        constexpr int s = bb * kBatch + u;
-        if constexpr (s < NST) {
-          constexpr int I = Plan::tab.sI[s];
-          acc[I] = mfma16x16x4(av[u], bv[u], acc[I]);
+      __syncthreads();  // staged operands visible to every wave
+      if (g + 1 < g1) prefetch(g + 1);  // in flight during the MFMAs below
+      const int cnt = WP::tab.cnt[q < 0 ? 0 : (q > P ? P : q)];
+      // batches of kBatch steps: all operand reads of a batch, then its
+      // MFMAs; a batch past the group's order is skipped
+      constexpr int kBatch = 16;
+      static_for<(NSW + kBatch - 1) / kBatch>([&](auto bb) {
+        if (bb * kBatch < cnt) {
+          T av[kBatch], bv[kBatch];
+          static_for<kBatch>([&](auto u) {
+            constexpr int i = bb * kBatch + u;
+            if constexpr (i < NSW) {
+              constexpr int s = WP::tab.s[i];
+              constexpr int K = Plan::tab.sK[s];
+              av[u] = sD[sidx[s * 64 + lane]];
+              bv[u] = sB[col * MSP + 4 * K + kk];
+            }
+          });
+          static_for<kBatch>([&](auto u) {
+            constexpr int i = bb * kBatch + u;
+            if constexpr (i < NSW) {
+              constexpr int s = WP::tab.s[i];
+              constexpr int I = Plan::tab.sI[s];
+              acc[I / W] = mfma16x16x4(av[u], bv[u], acc[I / W]);
+            }
+          });
         }
       });
-    });
-  }
-  const int cell = tile_cells[tile * kTile + col];
-  if (cell < 0) return;
-  T* Lc = L + (size_t)cell * LS;
-  static_for<NI>([&](auto I) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = 16 * I + acc_row<T>(kk, r);
-      if (n < LS) atomicAdd(&Lc[n], acc[I][r]);  // chunks of one tile add into the same L
     }
+    const int cell = tile_cells[tile * kTile + col];
+    if (cell < 0) return;
+    T* Lc = L + (size_t)cell * LS;
+    static_for<NIW>([&](auto ii) {
+      constexpr int I = wv + W * ii;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = 16 * I + acc_row<T>(kk, r);
+        if (n < LS) atomicAdd(&Lc[n], acc[ii][r]);  // chunks of one tile add into the same L
+      }
+    });
   });
 }
 

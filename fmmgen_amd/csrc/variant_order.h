@@ -116,16 +116,6 @@ __global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const doubl
   for (int j = 0; j < MS; ++j) Dtab[(size_t)k * MS + j] = (T)D[j];
 }
 
-// Staging of the source multipoles through LDS (coalesced rows) or direct
-// L1-cached loads of the B fragments; FMM_M2L_STAGE_B=0/1 selects.
-inline bool m2l_stage_b() {
-  static const int v = [] {
-    const char* e = getenv("FMM_M2L_STAGE_B");
-    return e ? atoi(e) : 1;
-  }();
-  return v != 0;
-}
-
 template <int P>
 void ensure_index_table(Tree& t, hipStream_t st) {
   if (t.sched.idxtab_order == P) return;
@@ -144,13 +134,11 @@ void m2l_plain(Tree& t, hipStream_t st) {
   S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
   const int grid = ((S.nchunks + 7) / 8) * 8;
-  auto launch = [&](auto kern) {
-    kern<<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
-                              S.chunk_g1.get(), S.group_key.get(), S.group_src.get(), S.key_dlevel.get(),
-                              S.Dtab64.get(), S.idxtab.get(), t.M.get(), t.L.get());
-  };
-  if (m2l_stage_b()) launch(k_m2l_gemm<P, SO, double, false, true>);
-  else launch(k_m2l_gemm<P, SO, double, false, false>);
+  constexpr int W = m2l_waves<P, SO, double>();
+  k_m2l_gemm<P, SO, double, false, W><<<grid, 64 * W, 0, st>>>(
+      S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
+      S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(), S.idxtab.get(), t.group_order_ptr(), t.M.get(),
+      t.L.get());
 }
 
 // L2L (example/calculate.cpp:219-234), x = child - parent, one thread per
@@ -244,13 +232,11 @@ void m2l_mixed(Tree& t, int mode, hipStream_t st) {
     return;
   }
   ensure_index_table<P>(t, st);
-  auto launch = [&](auto kern) {
-    kern<<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
-                              S.chunk_g1.get(), S.group_key.get(), S.group_src.get(), S.key_dlevel.get(),
-                              S.Dtab32.get(), S.idxtab.get(), t.Mf.get(), t.Lf.get());
-  };
-  if (m2l_stage_b()) launch(k_m2l_gemm<P, SO, float, true, true>);
-  else launch(k_m2l_gemm<P, SO, float, true, false>);
+  constexpr int W = m2l_waves<P, SO, float>();
+  k_m2l_gemm<P, SO, float, true, W><<<grid, 64 * W, 0, st>>>(
+      S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
+      S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), S.idxtab.get(), t.group_order_ptr(), t.Mf.get(),
+      t.Lf.get());
 }
 
 template <int P>
