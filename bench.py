@@ -8,7 +8,14 @@ reference's timer excludes it (example/main.cpp:138 vs 154-162).  The field
 error is ||F - F_ref||_2 / ||F_ref||_2 over sampled targets against an FP64
 direct sum on the GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--mode plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4] [--mode headline]
+
+The default (headline) run is the learned-order workload: N = 2^22 blobs,
+tolerance 1e-6.  It finds the lowest fixed order at which the plain FP64
+evaluation meets the tolerance on the sampled targets, times it, then times
+the learned evaluation (FP32 mixed precision, per-cell order from the
+network, sampled judge inside every evaluation) and reports the learned
+throughput as ``value`` with the plain one and the speed-up beside it.
 
 Prints one JSON line (rank 0).  A run asking for more than one GPU exits:
 this evaluator runs on one GPU.
@@ -30,8 +37,8 @@ CONFIGS = {
     "c1": dict(n=100_000, dist="uniform", order=4, theta=0.5, ncrit=64, so=0, ntargets=1000),
     "c2": dict(n=1 << 20, dist="uniform", order=8, theta=0.5, ncrit=64, so=0, ntargets=4096),
     "c3": dict(n=1_000_000, dist="plummer", order=6, theta=0.4, ncrit=32, so=1, ntargets=4096),
-    "c4": dict(n=1 << 22, dist="blobs", order=8, theta=0.5, ncrit=64, so=0, ntargets=4096),
-    "c5": dict(n=1 << 24, dist="mixed", order=8, theta=0.5, ncrit=128, so=0, ntargets=4096),
+    "c4": dict(n=1 << 22, dist="blobs", order=10, theta=0.5, ncrit=64, so=0, ntargets=4096),
+    "c5": dict(n=1 << 24, dist="mixed", order=10, theta=0.5, ncrit=128, so=0, ntargets=4096),
 }
 
 PEAK = {  # MI355X published peaks
@@ -43,23 +50,10 @@ P2P_FLOP_PER_PAIR = 20  # potential + field, charges
 
 
 def make_inputs(cfg, seed=1):
-    from tests import util
+    from fmmgen_amd import workloads as W
     n = cfg["n"]
-    rng = np.random.default_rng(seed)
-    if cfg["dist"] == "uniform":
-        pos = rng.random((n, 3))
-    elif cfg["dist"] == "plummer":
-        pos = util.plummer(n, seed, a=0.1)
-    elif cfg["dist"] == "blobs":
-        pos = util.blobs(n, seed)
-    else:  # half uniform, half Plummer a=0.05 truncated at r=1, centred in the cube
-        h = n // 2
-        pos = np.vstack([rng.random((h, 3)), util.plummer(n - h, seed, a=0.05) + 0.5])
-        pos = pos[rng.permutation(n)]
-    if cfg["so"] == 0:
-        S = rng.uniform(-1.0, 1.0, (n, 1))
-    else:
-        S = util.dipoles(n, seed + 1)
+    pos = W.positions(cfg["dist"], n, seed)
+    S = W.charges(n, seed + 1) if cfg["so"] == 0 else W.dipoles(n, seed + 1)
     return np.ascontiguousarray(pos), np.ascontiguousarray(S)
 
 
@@ -72,12 +66,13 @@ def cpu_baseline(cfg, budget_s=30.0):
     """The oracle (C++/OpenMP restatement of the example) on a bounded sample.
 
     Sample: the same distribution, order, theta and ncrit at n = min(N, 2^20)
-    (the whole workload up to 2^20 particles), one evaluation timed."""
+    (2^18 at order 8 and above, to keep the sample to tens of seconds), one
+    evaluation timed."""
     from oracle import oracle as O
     import ctypes
     so = cfg["so"]
     outsz = 4 if so == 0 else 3
-    n = min(cfg["n"], 1 << 20)
+    n = min(cfg["n"], (1 << 18) if cfg["order"] >= 8 else (1 << 20))
     sub = dict(cfg, n=n)
     pos, S = make_inputs(sub, seed=2)
     t0 = time.perf_counter()
@@ -92,15 +87,63 @@ def cpu_baseline(cfg, budget_s=30.0):
                       f"({t2 - t1:.1f} s; tree build {t1 - t0:.1f} s excluded)"}
 
 
+def timed(tree, F, mode, steps, warmup):
+    """(ms per evaluation, per-phase ms per evaluation) of ``steps`` timed evaluations."""
+    import torch
+    for _ in range(warmup):
+        tree.compute_field_fmm(F, mode=mode)
+    torch.cuda.synchronize()
+    tree.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tree.compute_field_fmm(F, mode=mode)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    phases = tree.timing_read()
+    tree.timing(False)
+    return (t1 - t0) * 1e3 / steps, {k: v[0] / steps for k, v in phases.items()}
+
+
+def field_error(F, Fref, tg, so):
+    import torch
+    Fs = F[tg]
+    fo = 1 if so == 0 else 0
+    ferr = (torch.linalg.norm(Fs[:, fo:] - Fref[:, fo:]) / torch.linalg.norm(Fref[:, fo:])).item()
+    perr = (torch.linalg.norm(Fs[:, 0] - Fref[:, 0]) / torch.linalg.norm(Fref[:, 0])).item() if so == 0 else None
+    return ferr, perr
+
+
+def roofline_of(phases, info, order, so, mode, m2l_macs=None):
+    """Roofline entry of the dominant kernel (P2P: VALU, M2L: MFMA); times are
+    hipEvent phase times on the evaluation stream."""
+    dom = max(("P2P", "M2L"), key=lambda k: phases[k])
+    if dom == "P2P":
+        flops = P2P_FLOP_PER_PAIR * info["p2p_particle_pairs"]
+        peak = PEAK["fp64_vector_TF"] if mode == "plain" else PEAK["fp32_vector_TF"]
+        bound = "valu"
+    else:
+        flops = 2 * m2l_macs if m2l_macs is not None else mac_flops_m2l(order, so) * info["n_m2l"]
+        peak = {"plain": PEAK["fp64_mfma_TF"], "f32": PEAK["fp32_mfma_TF"], "learned": PEAK["fp32_mfma_TF"],
+                "bf16x3": PEAK["bf16_mfma_TF"], "learned_bf16x3": PEAK["bf16_mfma_TF"]}[mode]
+        bound = "mfma"
+    achieved = flops / (phases[dom] * 1e-3) / 1e12
+    return {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": None, "avg_ms": phases[dom], "flop_per_launch": flops}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--mode", default="plain", choices=["plain", "f32", "bf16x3"])
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--mode", default="headline",
+                    choices=["headline", "plain", "f32", "bf16x3", "learned", "learned_bf16x3"])
     ap.add_argument("--order", type=int, default=None)
     ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--model", default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -116,80 +159,90 @@ def main():
         cfg["n"] = args.n
     n, so = cfg["n"], cfg["so"]
     outsz = 4 if so == 0 else 3
+    learned = args.mode in ("headline", "learned", "learned_bf16x3")
+    full = 12 if learned else cfg["order"]
     dev = torch.device("cuda:0")
     pos_h, S_h = make_inputs(cfg)
     pos = torch.from_numpy(pos_h).to(dev)
     S = torch.from_numpy(S_h).to(dev)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    tree = build_tree(pos, S, n, cfg["ncrit"], cfg["order"], cfg["theta"], source_order=so, potential=(so == 0))
+    tree = build_tree(pos, S, n, cfg["ncrit"], full, cfg["theta"], source_order=so, potential=(so == 0))
     torch.cuda.synchronize()
     build_ms = (time.perf_counter() - t0) * 1e3
     info = tree.info()
     print(f"# tree: {info} build {build_ms:.1f} ms", file=sys.stderr, flush=True)
 
-    F = torch.zeros(n, outsz, dtype=torch.float64, device=dev)
-    for _ in range(args.warmup):
-        tree.compute_field_fmm(F, mode=args.mode)
-    torch.cuda.synchronize()
-    tree.timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tree.compute_field_fmm(F, mode=args.mode)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    phases = tree.timing_read()
-    tree.timing(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    print(f"# {ms_per_step:.3f} ms/eval; phases (ms/eval): "
-          + ", ".join(f"{k} {v[0] / max(v[1], 1):.3f}" for k, v in phases.items()), file=sys.stderr, flush=True)
-
-    # accuracy on sampled targets
     rng = np.random.default_rng(7)
     tg = torch.from_numpy(rng.choice(n, min(cfg["ntargets"], n), replace=False).astype(np.int64)).to(dev)
     Fref = direct_sum(pos, S, tg, source_order=so, outsz=outsz)
-    Fs = F[tg]
-    fo = 1 if so == 0 else 0
-    ferr = (torch.linalg.norm(Fs[:, fo:] - Fref[:, fo:]) / torch.linalg.norm(Fref[:, fo:])).item()
-    perr = (torch.linalg.norm(Fs[:, 0] - Fref[:, 0]) / torch.linalg.norm(Fref[:, 0])).item() if so == 0 else None
-
-    # roofline of the dominant kernel (phase timed with hipEvents on the evaluation stream)
-    avg = {k: v[0] / max(v[1], 1) for k, v in phases.items()}
-    dom = max(("P2P", "M2L"), key=lambda k: avg[k])
-    if dom == "P2P":
-        flops = P2P_FLOP_PER_PAIR * info["p2p_particle_pairs"]
-        peak = PEAK["fp64_vector_TF"] if args.mode == "plain" else PEAK["fp32_vector_TF"]
-        bound = "valu"
+    F = torch.zeros(n, outsz, dtype=torch.float64, device=dev)
+    extra = {}
+    if args.mode == "headline":
+        # the method without the network: plain FP64 at the lowest fixed order meeting tol
+        pfix = None
+        for p in range(6, 13):
+            tree.set_order(p)
+            tree.compute_field_fmm(F, mode="plain")
+            e, _ = field_error(F, Fref, tg, so)
+            print(f"# plain FP64 order {p}: field error {e:.3e}", file=sys.stderr, flush=True)
+            if e <= args.tol:
+                pfix = p
+                break
+        pfix = pfix or 12
+        tree.set_order(pfix)
+        ms_plain, ph_plain = timed(tree, F, "plain", args.steps, args.warmup)
+        e_plain, _ = field_error(F, Fref, tg, so)
+        print(f"# plain FP64 order {pfix}: {ms_plain:.3f} ms/eval", file=sys.stderr, flush=True)
+        extra["plain_f64"] = {"order": pfix, "ms_per_step": ms_plain, "value": n / (ms_plain * 1e-3),
+                              "field_rel_err": e_plain, "phase_ms": ph_plain}
+        tree.set_order(full)
+        mode = "learned"
     else:
-        flops = mac_flops_m2l(cfg["order"], so) * info["n_m2l"]
-        peak = {"plain": PEAK["fp64_mfma_TF"], "f32": PEAK["fp32_mfma_TF"], "bf16x3": PEAK["bf16_mfma_TF"]}[args.mode]
-        bound = "mfma"
-    achieved = flops / (avg[dom] * 1e-3) / 1e12
-    roofline = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "avg_ms": avg[dom], "flop_per_launch": flops}
+        mode = args.mode
+    if learned:
+        from fmmgen_amd import learned as LN
+        params, margin, _ = LN.load(args.model or LN.MODEL_PATH)
+        tree.set_model(params, args.tol, margin)
+        extra["model_margin"] = margin
+    ms_per_step, avg = timed(tree, F, mode, args.steps, args.warmup)
+    print(f"# {mode}: {ms_per_step:.3f} ms/eval; phases (ms/eval): "
+          + ", ".join(f"{k} {v:.3f}" for k, v in avg.items()), file=sys.stderr, flush=True)
+    ferr, perr = field_error(F, Fref, tg, so)
+    stats = tree.learned_stats() if learned else None
+    if stats:
+        extra["learned"] = stats
+        print(f"# learned stats {stats}", file=sys.stderr, flush=True)
+    roofline = roofline_of(avg, info, cfg["order"], so, mode, stats["m2l_macs"] if stats else None)
 
     base = None
     if not args.no_cpu_baseline:
         try:
-            base = cpu_baseline(cfg)
+            bcfg = dict(cfg, order=extra["plain_f64"]["order"]) if "plain_f64" in extra else cfg
+            base = cpu_baseline(bcfg)
         except Exception as e:  # the baseline is reported, never the target
             base = {"value": None, "error": str(e)}
 
     value = n / (ms_per_step * 1e-3)
+    dtype = {"plain": "f64", "f32": "f32", "bf16x3": "bf16x3+f32", "learned": "f32",
+             "learned_bf16x3": "bf16x3+f32"}[mode]
+    workload = (f"{args.config}: N={n} {cfg['dist']}, theta {cfg['theta']}, ncrit {cfg['ncrit']}, "
+                f"source_order {so}, " + (f"learned order 2-{full}, tol {args.tol:g}" if learned
+                                          else f"order {cfg['order']}"))
     out = {
         "metric": "FMM particle-evals/sec at 1e-6 rel field err, mixed+learned vs FP64 plain",
         "value": value, "unit": "particle-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.mode == "plain" else ("f32" if args.mode == "f32" else "bf16x3+f32"),
-        "data": "synthetic",
-        "config": {"workload": f"{args.config}: N={n} {cfg['dist']}, order {cfg['order']}, theta {cfg['theta']}, "
-                               f"ncrit {cfg['ncrit']}, source_order {so}", "mode": args.mode,
-                   "n": n, "order": cfg["order"], "theta": cfg["theta"], "ncrit": cfg["ncrit"]},
+        "dtype": dtype, "data": "synthetic",
+        "config": {"workload": workload, "mode": mode, "n": n, "order": full if learned else cfg["order"],
+                   "theta": cfg["theta"], "ncrit": cfg["ncrit"]},
         "field_rel_err": ferr, "potential_rel_err": perr, "meets_1e-6": bool(ferr <= 1e-6),
         "tree_build_ms": build_ms, "phase_ms": avg, "tree": info,
         "roofline": roofline, "cpu_baseline": base,
     }
+    if "plain_f64" in extra:
+        out["speedup_vs_plain_f64"] = extra["plain_f64"]["ms_per_step"] / ms_per_step
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

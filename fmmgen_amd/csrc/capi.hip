@@ -6,6 +6,8 @@
 
 #include "../../include/fmm_hip.h"
 #include "evaluate.h"
+#include "learned.h"
+#include "mlp.h"
 #include "fmm_common.h"
 
 struct fmm_tree_s {
@@ -209,6 +211,13 @@ int fmm_tree_set_sources(fmm_tree_t tree, const double* d_S) {
 int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, void* stream) {
   return guarded([&] {
     FMM_CHECK(tree && d_F, kInvalidArgument, "NULL argument");
+    if (mode == FMM_MODE_LEARNED_F32 || mode == FMM_MODE_LEARNED_BF16X3) {
+      FMM_CHECK(phase_mask == kPhaseAll, kInvalidArgument, "the learned modes run every phase");
+      FMM_CHECK(tree->t.so == 0, kUnsupported, "the learned modes need source order 0");
+      tree->v->evaluate_learned(tree->t, mode, d_F, static_cast<hipStream_t>(stream));
+      FMM_HIP(hipGetLastError());
+      return;
+    }
     FMM_CHECK(mode == FMM_MODE_PLAIN_F64 || mode == FMM_MODE_MIXED_F32 || mode == FMM_MODE_MIXED_BF16X3,
               kInvalidArgument, "unknown evaluation mode");
     FMM_CHECK((phase_mask & ~kPhaseAll) == 0, kInvalidArgument, "unknown phase bits");
@@ -324,6 +333,121 @@ int fmm_export_expansions(fmm_tree_t tree, double* h_M, double* h_L) {
     FMM_CHECK(!h_L || t.L.n >= ls, kInvalidArgument, "no FP64 expansions: evaluate first");
     if (h_M) FMM_HIP(hipMemcpy(h_M, t.M.get(), sizeof(double) * ms, hipMemcpyDeviceToHost));
     if (h_L) FMM_HIP(hipMemcpy(h_L, t.L.get(), sizeof(double) * ls, hipMemcpyDeviceToHost));
+  });
+}
+
+// ------------------------------------------------------------ learned order
+int fmm_mlp_forward(const float* d_X, int B, const float* d_params, float* d_Y, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(B >= 0 && (B == 0 || (d_X && d_params && d_Y)), kInvalidArgument, "NULL argument");
+    mlp_forward(B, d_X, d_params, d_Y, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
+int fmm_mlp_infer_f16(const float* d_X, int B, const float* d_params, float* d_Y, int* d_order, double tol,
+                      double margin, int min_order, int max_order, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(B >= 0 && (B == 0 || (d_X && d_params)), kInvalidArgument, "NULL argument");
+    FMM_CHECK(tol > 0 && min_order <= max_order, kInvalidArgument, "bad tolerance or order range");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    DevBuf<_Float16> h;
+    h.alloc(kMlpParams);
+    mlp_to_half(d_params, h.get(), st);
+    mlp_infer_f16(B, d_X, h.get(), d_Y, d_order, (float)log10(tol), (float)margin, min_order, max_order, st);
+    FMM_HIP(hipStreamSynchronize(st));
+  });
+}
+
+size_t fmm_mlp_work_floats(int B) { return B > 0 ? mlp_work_floats(B) : 0; }
+
+int fmm_mlp_grad(const float* d_X, const float* d_T, const float* d_mask, int B, const float* d_params,
+                 float* d_grad, float* d_loss, float* d_work, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(d_X && d_T && d_mask && d_params && d_grad && d_loss && d_work, kInvalidArgument, "NULL argument");
+    mlp_grad(B, d_X, d_T, d_mask, d_params, d_grad, d_loss, d_work, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
+int fmm_mlp_adam(const float* d_grad, float* d_params, float* d_m, float* d_v, float lr, float beta1, float beta2,
+                 float eps, int step, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(d_grad && d_params && d_m && d_v, kInvalidArgument, "NULL argument");
+    mlp_adam(d_grad, d_params, d_m, d_v, lr, beta1, beta2, eps, step, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
+int fmm_tree_set_order(fmm_tree_t tree, int order) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    FMM_CHECK(order >= tree->v->min_order && order < tree->v->max_order, kUnsupported,
+              "order " + std::to_string(order) + " not compiled for this variant");
+    tree->t.order = order;
+  });
+}
+
+int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(tree && d_feat, kInvalidArgument, "NULL argument");
+    Tree& t = tree->t;
+    FMM_CHECK(t.Fp2p.get() && t.pS.get(), kInvalidArgument, "no evaluation with P2P has run on this tree");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    leaf_features(t, sourcesize(t.so), t.outsz, t.outsz == 4 ? 1 : 0, st);
+    if (t.nleaves > 0)
+      FMM_HIP(hipMemcpyAsync(d_feat, t.leaf_feat.get(), sizeof(float) * t.nleaves * kMlpF,
+                             hipMemcpyDeviceToDevice, st));
+    FMM_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int fmm_tree_leaf_error(fmm_tree_t tree, const double* d_F, const double* d_Fref, double* d_err, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(tree && d_F && d_Fref && d_err, kInvalidArgument, "NULL argument");
+    Tree& t = tree->t;
+    leaf_error(t, d_F, d_Fref, t.outsz, t.outsz == 4 ? 1 : 0, d_err, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
+int fmm_tree_leaves(fmm_tree_t tree, long long* h_leaves) {
+  return guarded([&] {
+    FMM_CHECK(tree && h_leaves, kInvalidArgument, "NULL argument");
+    const Tree& t = tree->t;
+    std::vector<int> l(t.nleaves);
+    if (t.nleaves > 0) FMM_HIP(hipMemcpy(l.data(), t.leaves.get(), sizeof(int) * t.nleaves, hipMemcpyDeviceToHost));
+    for (int i = 0; i < t.nleaves; ++i) h_leaves[i] = l[i];
+  });
+}
+
+int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, double margin) {
+  return guarded([&] {
+    FMM_CHECK(tree && d_params, kInvalidArgument, "NULL argument");
+    FMM_CHECK(tol > 0 && tol < 1, kInvalidArgument, "tolerance must be in (0, 1)");
+    Tree& t = tree->t;
+    t.mlp_half.alloc(kMlpParams);
+    mlp_to_half(d_params, t.mlp_half.get(), nullptr);
+    FMM_HIP(hipStreamSynchronize(nullptr));
+    t.has_model = true;
+    t.tol = tol;
+    t.margin = margin;
+  });
+}
+
+int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats) {
+  return guarded([&] {
+    FMM_CHECK(tree && h_stats, kInvalidArgument, "NULL argument");
+    for (int k = 0; k < 8; ++k) h_stats[k] = tree->t.lstats[k];
+  });
+}
+
+int fmm_tree_leaf_orders(fmm_tree_t tree, int* h_order) {
+  return guarded([&] {
+    FMM_CHECK(tree && h_order, kInvalidArgument, "NULL argument");
+    const Tree& t = tree->t;
+    FMM_CHECK(t.leaf_q.get() != nullptr, kInvalidArgument, "no learned evaluation has run on this tree");
+    if (t.nleaves > 0) FMM_HIP(hipMemcpy(h_order, t.leaf_q.get(), sizeof(int) * t.nleaves, hipMemcpyDeviceToHost));
   });
 }
 

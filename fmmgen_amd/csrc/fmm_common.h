@@ -90,7 +90,9 @@ enum PhaseTimerId : int {
   kTP2P = 4,
   kTL2L = 5,
   kTL2P = 6,    // L2P plus the final per-target sum
-  kNumPhaseTimers = 8,
+  kTOrder = 7,  // learned order: features, network inference, cell and tile orders
+  kTJudge = 8,  // learned order: sampled direct-sum judge and reruns
+  kNumPhaseTimers = 10,
 };
 
 // ------------------------------------------------------------ M2L schedule
@@ -113,8 +115,12 @@ struct M2LSchedule {
   DevBuf<double> Dtab64;
   DevBuf<float> Dtab32;
   // (step, lane) -> D slot table of the GEMM plan, uploaded once per order
-  DevBuf<short> idxtab, idxtab32;
-  int idxtab_order = -1, idxtab32_order = -1;
+  static constexpr int kMaxOrder = 16;
+  DevBuf<short> idxtab[kMaxOrder], idxtab32[kMaxOrder];
+  bool idx_ready[kMaxOrder] = {false}, idx32_ready[kMaxOrder] = {false};
+  // learned order: chunks sorted by their tile's order, and the start of
+  // every order's range (kMaxOrder + 1 entries)
+  DevBuf<int> class_chunks;
   // work items: chunks of at most kGroupsPerChunk groups of one tile
   int nchunks = 0;
   DevBuf<int> chunk_tile, chunk_g0, chunk_g1;
@@ -162,15 +168,34 @@ struct Tree {
   // offset in that range, source count); split leaves accumulate with
   // atomics into a zeroed output.
   static constexpr int kP2PSegment = 8192;
-  // Per M2L group truncation order (learned order); empty: every group at
-  // the tree's order.
-  DevBuf<signed char> group_order;
-  bool use_group_order = false;
-  const signed char* group_order_ptr() const { return use_group_order ? group_order.get() : nullptr; }
   long long n_p2p_items = 0;
   int p2p_split = 0;  // some leaf has more than one item
   DevBuf<int> p2p_item_leaf, p2p_item_r, p2p_item_o, p2p_item_n;
   M2LSchedule sched;
+
+  // learned order: per leaf features and order, per cell order (the
+  // largest order of the leaves below), per M2L tile order (the largest
+  // order of its 16 target cells; the tile's groups are truncated there)
+  DevBuf<float> leaf_feat;         // [nleaves * kMlpF]
+  DevBuf<int> leaf_q, cell_q;      // [nleaves], [ncells]
+  DevBuf<double> cell_qabs;        // [ncells] sum of |source strength| below the cell
+  DevBuf<double> cell_acc;         // [ncells * 7] list sums, accumulated down the tree
+  DevBuf<double> fscale;           // [2] sum of squares of the near field, rms
+  DevBuf<signed char> tile_order;  // [ntiles]
+  bool use_tile_order = false;
+  const signed char* tile_order_ptr() const { return use_tile_order ? tile_order.get() : nullptr; }
+  // M2L by tile-order classes (set by the learned evaluation): called by
+  // the far field of order P after the D table is built at order P
+  void (*m2l_classes)(Tree&, int mode, int P, hipStream_t) = nullptr;
+  DevBuf<_Float16> mlp_half;       // network parameters in FP16
+  bool has_model = false;
+  double tol = 1e-6, margin = 0.0;
+  // sampled judge: targets (sorted slots), their FP64 direct sums, results
+  int judge_n = 0;
+  DevBuf<int> judge_slot;
+  DevBuf<double> judge_Fd, judge_acc;  // [judge_n * OUT], [4]
+  DevBuf<int> judge_fail;              // [1 + ncells] failing count, per cell flag
+  double lstats[8] = {0};
 
   // expansions, [ncells * Msize] and [ncells * Lsize] in FP64
   DevBuf<double> M, L;

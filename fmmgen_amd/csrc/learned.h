@@ -1,0 +1,46 @@
+// Learned per-cell expansion order (no counterpart in the reference).
+//
+// Per leaf, eight features describe how strongly the far field reaches the
+// leaf: with rho = (rmax_A + rmax_B) / R and w = sum|S|_B / R^2 over every
+// M2L pair (B -> A) of the leaf and of its ancestors,
+//   f0..f3  log10(sum w rho^k / F_near), k = 3, 6, 9, 12
+//   f4      largest rho
+//   f5      log10(1 + number of those pairs)
+//   f6      log2(particles in the leaf)
+//   f7      log10(sum w / F_near)
+// where F_near is the rms near-field (P2P) magnitude of the evaluation.  The
+// network maps them to the predicted rms far-field error of the leaf per
+// order (relative to the rms field of all targets); a cell's
+// order is the largest order of the leaves below it, and an M2L tile runs
+// at the largest order of its target cells.
+#pragma once
+#include "fmm_common.h"
+#include "mlp.h"
+
+namespace fmm {
+
+// Features of every leaf (canonical leaf order) into t.leaf_feat; uses the
+// sorted near field t.Fp2p (outsz values per target, field components
+// from fo) and sorted source strengths t.pS (ss per particle).
+void leaf_features(Tree& t, int ss, int outsz, int fo, hipStream_t st);
+// Root-mean-square |F - Fref| (field components) over the particles of
+// each leaf; F and Fref in caller order.
+void leaf_error(Tree& t, const double* F, const double* Fref, int outsz, int fo, double* err, hipStream_t st);
+// Cell and tile orders from t.leaf_q; returns the largest order (the
+// root's), read back to the host.
+int orders_from_leaves(Tree& t, hipStream_t st);
+// Leaves holding a failing judge target (flag set in t.judge_fail[1 + c])
+// move to order qfull.
+void raise_failing_leaves(Tree& t, int qfull, hipStream_t st);
+// M2L chunks sorted by their tile's order into t.sched.class_chunks;
+// start[q] .. start[q + 1] is order q's range (start has
+// M2LSchedule::kMaxOrder + 1 entries, host).
+void chunk_classes(Tree& t, hipStream_t st, int* start);
+// Every leaf moves to order qfull.
+void raise_all_leaves(Tree& t, int qfull, hipStream_t st);
+// Sum over M2L pairs of C(q + 6, 6) at their tile's order (the useful
+// multiply-adds of the truncated M2L, into *macs_out), divided by the same
+// sum at order pref; host result.
+double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out = nullptr);
+
+}  // namespace fmm

@@ -170,9 +170,12 @@ __device__ __forceinline__ void dispatch_wave(int w, F&& f) {
   });
 }
 
-// group_order (optional, per group): the order q <= P the group's pairs are
-// truncated at: D entries of degree > q are zeroed and the steps whose
-// lowest degree exceeds q are not issued.
+// tile_order (optional, per tile): the order q <= P the pairs of the tile's
+// groups are truncated at: D entries of degree > q are zeroed and the
+// steps whose lowest degree exceeds q are not issued.  chunk_list
+// (optional): the chunks to run (else 0 .. nchunks-1).  mstride / lstride:
+// row strides of M and the D table / of L (the sizes of a higher order
+// when the expansions are stored for one: the layout is a prefix).
 template <int P, int SO, typename T, bool SCALED, int W>
 __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, const int* __restrict__ tile_cells,
                                                      const int* __restrict__ chunk_tile,
@@ -182,7 +185,8 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
                                                      const int* __restrict__ group_src,
                                                      const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
                                                      const short* __restrict__ idxtab,
-                                                     const signed char* __restrict__ group_order,
+                                                     const signed char* __restrict__ tile_order,
+                                                     const int* __restrict__ chunk_list, int mstride, int lstride,
                                                      const T* __restrict__ M, T* __restrict__ L) {
   using Plan = M2LPlan<P, SO>;
   constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NST = Plan::NSTEP;
@@ -196,8 +200,9 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   __shared__ T sB[kTile * MSP + 4];       // tail slots stay zero
   __shared__ signed char sdeg[MS];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int chunk = xcd_tile(blockIdx.x, nblocks);
-  if (chunk >= nchunks) return;  // uniform over the block
+  const int ci = xcd_tile(blockIdx.x, nblocks);
+  if (ci >= nchunks) return;  // uniform over the block
+  const int chunk = chunk_list ? chunk_list[ci] : ci;
   const int tile = chunk_tile[chunk];
   for (int i = tid; i < NST * 64; i += 64 * W) {
     const short v = idxtab[i];
@@ -208,6 +213,8 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   for (int j = tid; j < MS; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
+  const int q = tile_order ? (int)tile_order[tile] : P;
+  const int qc = q < 0 ? 0 : (q > P ? P : q);
 
   dispatch_wave<W>(w, [&](auto WV) {
     constexpr int wv = WV;
@@ -223,7 +230,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        rD[i] = j < MS ? Dtab[(size_t)key * MS + j] : T(0);
+        rD[i] = j < MS ? Dtab[(size_t)key * mstride + j] : T(0);
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c) {
@@ -231,19 +238,18 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 #pragma unroll
         for (int i = 0; i < NJ; ++i) {
           const int j = 64 * i + lane;
-          rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : T(0);
+          rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * mstride + j] : T(0);
         }
       }
     };
     if (g0 < g1) prefetch(g0);
     for (int g = g0; g < g1; ++g) {
-      const int q = group_order ? (int)group_order[g] : P;
       const int dl = SCALED ? key_dlevel[group_key[g]] : 0;
       __syncthreads();  // previous group finished reading sD / sB
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        if (j < MS) sD[j] = sdeg[j] <= q ? rD[i] : T(0);
+        if (j < MS) sD[j] = sdeg[j] <= qc ? rD[i] : T(0);
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c)
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
         }
       __syncthreads();  // staged operands visible to every wave
       if (g + 1 < g1) prefetch(g + 1);  // in flight during the MFMAs below
-      const int cnt = WP::tab.cnt[q < 0 ? 0 : (q > P ? P : q)];
+      const int cnt = WP::tab.cnt[qc];
       // batches of kBatch steps: all operand reads of a batch, then its
       // MFMAs; a batch past the group's order is skipped
       constexpr int kBatch = 16;
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
     }
     const int cell = tile_cells[tile * kTile + col];
     if (cell < 0) return;
-    T* Lc = L + (size_t)cell * LS;
+    T* Lc = L + (size_t)cell * lstride;
     static_for<NIW>([&](auto ii) {
       constexpr int I = wv + W * ii;
 #pragma unroll
@@ -384,6 +390,8 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
                                                    const int* __restrict__ group_src,
                                                    const int* __restrict__ key_dlevel,
                                                    const float* __restrict__ Dtab, const short* __restrict__ idxtab,
+                                                   const signed char* __restrict__ tile_order,
+                                                   const int* __restrict__ chunk_list, int mstride, int lstride,
                                                    const float* __restrict__ M, float* __restrict__ L) {
   using Plan = M2LPlan32<P, SO>;
   constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NK = Plan::NK, NST = Plan::NSTEP;
@@ -395,8 +403,9 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
   __shared__ float sB[kTile * RS];
   __shared__ signed char sdeg[MS];
   const int lane = threadIdx.x;
-  const int chunk = xcd_tile(blockIdx.x, nblocks);
-  if (chunk >= nchunks) return;
+  const int ci = xcd_tile(blockIdx.x, nblocks);
+  if (ci >= nchunks) return;
+  const int chunk = chunk_list ? chunk_list[ci] : ci;
   const int tile = chunk_tile[chunk];
   for (int i = lane; i < NST * 8 * 64; i += 64) {
     const short v = idxtab[i];
@@ -410,13 +419,15 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
   for (int I = 0; I < NI; ++I) acc[I] = V4{0, 0, 0, 0};
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
+  const int q = tile_order ? (int)tile_order[tile] : P;
+  const int qc = q < 0 ? 0 : (q > P ? P : q);
   float rD[NJ], rB[kTile][NJ];
   auto prefetch = [&](int g) {
     const int key = group_key[g];
 #pragma unroll
     for (int i = 0; i < NJ; ++i) {
       const int j = 64 * i + lane;
-      rD[i] = j < MS ? Dtab[(size_t)key * MS + j] : 0.f;
+      rD[i] = j < MS ? Dtab[(size_t)key * mstride + j] : 0.f;
     }
 #pragma unroll
     for (int c = 0; c < kTile; ++c) {
@@ -424,7 +435,7 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
 #pragma unroll
       for (int i = 0; i < NJ; ++i) {
         const int j = 64 * i + lane;
-        rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * MS + j] : 0.f;
+        rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * mstride + j] : 0.f;
       }
     }
   };
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
 #pragma unroll
     for (int i = 0; i < NJ; ++i) {
       const int j = 64 * i + lane;
-      if (j < MS) sD[j] = pack_split(rD[i]);
+      if (j < MS) sD[j] = sdeg[j] <= qc ? pack_split(rD[i]) : 0u;
     }
 #pragma unroll
     for (int c = 0; c < kTile; ++c)
@@ -448,6 +459,8 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
     if (g + 1 < g1) prefetch(g + 1);
     static_for<NK>([&](auto K) {
       constexpr int nI = Plan::tab.nI[K];
+      constexpr int dK = slot_degree(32 * K + Plan::OFF);
+      if (dK > qc) return;  // slices past the tile's order
       float bv[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) bv[j] = sB[col * RS + 32 * K + 8 * kk + j];
@@ -455,6 +468,7 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
       split_bf16(bv, bhi, blo);
       static_for<nI>([&](auto I) {
         constexpr int s = Plan::tab.base[K] + I;
+        if (slot_degree(16 * I) + dK > qc) return;
         unsigned aw[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) aw[i] = sD[sidx[(s * 8 + i) * 64 + lane]];
@@ -468,7 +482,7 @@ __global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, con
   }
   const int cell = tile_cells[tile * kTile + col];
   if (cell < 0) return;
-  float* Lc = L + (size_t)cell * LS;
+  float* Lc = L + (size_t)cell * lstride;
   static_for<NI>([&](auto I) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {

@@ -2,6 +2,7 @@
 // near field, the FP64 direct sum, order dispatch and the variant table.
 #pragma once
 
+#include "learned.h"
 #include "variant_decl.h"
 
 namespace fmm {
@@ -57,6 +58,89 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
   if (i >= n) return;
   fmmgen_ops::P2P<double>(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], in + (size_t)i * in_stride,
                           out + (size_t)i * out_stride);
+}
+
+// Sampled judge of the learned order: FP64 direct sum (self excluded) at
+// sorted slots, one block per target.
+__global__ void k_judge_direct(int n, const double* __restrict__ px, const double* __restrict__ py,
+                               const double* __restrict__ pz, const double* __restrict__ pS,
+                               const int* __restrict__ slots, int J, double* __restrict__ Fd) {
+  const int t = blockIdx.x;
+  if (t >= J) return;
+  const int i = slots[t];
+  const double xi = px[i], yi = py[i], zi = pz[i];
+  double acc[OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) acc[k] = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    if (j == i) continue;
+    fmmgen_ops::P2P<double>(xi - px[j], yi - py[j], zi - pz[j], pS + (size_t)j * SS, acc);
+  }
+  __shared__ double red[256][OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) red[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < OUT; ++k) red[threadIdx.x][k] += red[threadIdx.x + s][k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) Fd[(size_t)t * OUT + k] = red[0][k];
+}
+
+// Judge check (one block): a target fails when |F - Fd| over the field
+// components exceeds tol times the rms |Fd| of the sample; the leaf holding
+// it is flagged.  acc = (sum |F - Fd|^2, sum |Fd|^2).
+__global__ void k_judge_check(int J, const int* __restrict__ slots, const int* __restrict__ perm,
+                              const int* __restrict__ leaf_of, const double* __restrict__ F,
+                              const double* __restrict__ Fd, double tol, double* __restrict__ acc,
+                              int* __restrict__ fail) {
+  constexpr int FO = OUT == 4 ? 1 : 0;
+  __shared__ double r1[256], r2[256];
+  double e2 = 0.0, f2 = 0.0;
+  for (int t = threadIdx.x; t < J; t += blockDim.x) {
+    const size_t i = (size_t)perm[slots[t]];
+    for (int k = FO; k < OUT; ++k) {
+      const double d = F[i * OUT + k] - Fd[(size_t)t * OUT + k];
+      e2 += d * d;
+      f2 += Fd[(size_t)t * OUT + k] * Fd[(size_t)t * OUT + k];
+    }
+  }
+  r1[threadIdx.x] = e2;
+  r2[threadIdx.x] = f2;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      r1[threadIdx.x] += r1[threadIdx.x + w];
+      r2[threadIdx.x] += r2[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  const double lim = tol * sqrt(r2[0] / (double)(J > 0 ? J : 1));
+  if (threadIdx.x == 0) {
+    acc[0] = r1[0];
+    acc[1] = r2[0];
+  }
+  for (int t = threadIdx.x; t < J; t += blockDim.x) {
+    const size_t i = (size_t)perm[slots[t]];
+    double d2 = 0.0;
+    for (int k = FO; k < OUT; ++k) {
+      const double d = F[i * OUT + k] - Fd[(size_t)t * OUT + k];
+      d2 += d * d;
+    }
+    if (sqrt(d2) > lim) {
+      atomicAdd(fail, 1);
+      fail[1 + leaf_of[slots[t]]] = 1;
+    }
+  }
+}
+
+__global__ void k_fill_int(long long n, int* __restrict__ a) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
 // P2P near field (example/calculate.cpp:24-40, 208-216), one wavefront per
@@ -223,6 +307,10 @@ void near_field(Tree& t, int mode, hipStream_t st) {
 
 template <int P>
 struct OrderTable {
+  static void m2l(int order, Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hipStream_t st) {
+    if (order == P) m2l_class<P>(t, mode, chunks, nch, ms, ls, st);
+    else OrderTable<P + 1>::m2l(order, t, mode, chunks, nch, ms, ls, st);
+  }
   static void far(int order, Tree& t, int mode, int mask, hipStream_t st) {
     if (order == P) far_field_order<P>(t, mode, mask, st);
     else OrderTable<P + 1>::far(order, t, mode, mask, st);
@@ -239,6 +327,9 @@ struct OrderTable {
 };
 template <>
 struct OrderTable<PMAX> {
+  static void m2l(int, Tree&, int, const int*, int, int, int, hipStream_t) {
+    throw Error{kUnsupported, "order not compiled"};
+  }
   static void far(int, Tree&, int, int, hipStream_t) { throw Error{kUnsupported, "order not compiled"}; }
   static void l2p(int, Tree&, int, int, double*, hipStream_t) { throw Error{kUnsupported, "order not compiled"}; }
   static void op(int, int, int, const double*, const double*, int, double*, int, hipStream_t) {
@@ -265,6 +356,122 @@ void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
   OrderTable<PMIN>::far(t.order, t, mode, mask, st);
   if (mask & kPhaseP2P) near_field(t, mode, st);
   OrderTable<PMIN>::l2p(t.order, t, mode, mask, F, st);
+}
+
+// Learned-order evaluation: FP32 P2P, leaf features, FP16 network
+// inference -> leaf orders -> cell and tile orders, mixed-precision far
+// field at the largest order with every M2L tile truncated at its own
+// order, then the sampled judge: an FP64 direct sum at kJudgeTargets
+// targets.  When the sample's relative field error exceeds the tolerance,
+// the leaves holding a target whose error exceeds tol * rms |F| rerun at
+// the tree's order (all leaves when more than one target in
+// kJudgeDistrust does).
+// mode 3: FP32 M2L, mode 4: BF16x3 M2L.
+// M2L of the learned evaluation: chunks grouped by their tile's order,
+// each group launched with the kernel of its own order on the expansions
+// of the far field's order P (row strides of order P).
+void m2l_by_class(Tree& t, int mode, int P, hipStream_t st) {
+  int start[M2LSchedule::kMaxOrder + 1];
+  chunk_classes(t, st, start);
+  const int ms = msize(P, SO), ls = lsize(P, SO);
+  for (int q = PMIN; q <= P && q < M2LSchedule::kMaxOrder; ++q) {
+    const int nch = start[q + 1] - start[q];
+    if (nch > 0) OrderTable<PMIN>::m2l(q, t, mode, t.sched.class_chunks.get() + start[q], nch, ms, ls, st);
+  }
+}
+
+constexpr int kJudgeTargets = 256;
+constexpr int kJudgeDistrust = 64;
+
+void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
+  FMM_CHECK(t.has_model, kInvalidArgument, "no order model set on the tree");
+  constexpr int FO = OUT == 4 ? 1 : 0;
+  const int n = (int)t.n, nl = t.nleaves;
+  const int fmode = mode == 4 ? 2 : 1;
+  t.Fp2p.alloc((size_t)std::max(n, 1) * OUT);
+  t.pS.alloc((size_t)std::max(n, 1) * SS);
+  {
+    PhaseTimer pt(t, kTPrep, st);
+    k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+  }
+  near_field(t, 1, st);
+  int peff;
+  {
+    PhaseTimer pt(t, kTOrder, st);
+    leaf_features(t, SS, OUT, FO, st);
+    t.leaf_q.alloc(std::max(nl, 1));
+    mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), nullptr, t.leaf_q.get(), (float)log10(t.tol),
+                  (float)t.margin, std::max(PMIN, kMlpOrder0), t.order, st);
+    peff = orders_from_leaves(t, st);
+  }
+  peff = std::max(std::min(peff, t.order), PMIN);
+  struct Restore {  // the tile orders apply to this evaluation only
+    Tree& t;
+    ~Restore() {
+      t.use_tile_order = false;
+      t.m2l_classes = nullptr;
+    }
+  } restore{t};
+  t.use_tile_order = true;
+  t.m2l_classes = &m2l_by_class;
+  auto far = [&](int p) {
+    OrderTable<PMIN>::far(p, t, fmode, kPhaseAll & ~kPhaseP2P, st);
+    OrderTable<PMIN>::l2p(p, t, fmode, kPhaseAll, F, st);
+  };
+  far(peff);
+  // judge
+  const int J = std::min(kJudgeTargets, n);
+  int fails = 0, reruns = 0;
+  double jerr = 0.0;
+  if (t.judge_n != J) {
+    std::vector<int> h(J);
+    for (int i = 0; i < J; ++i) h[i] = (int)(((long long)(2 * i + 1) * n) / (2LL * J));
+    t.judge_slot.alloc(std::max(J, 1));
+    FMM_HIP(hipMemcpyAsync(t.judge_slot.get(), h.data(), sizeof(int) * J, hipMemcpyHostToDevice, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    t.judge_n = J;
+  }
+  t.judge_Fd.alloc((size_t)std::max(J, 1) * OUT);
+  t.judge_acc.alloc(4);
+  t.judge_fail.alloc((size_t)t.ncells + 1);
+  auto check = [&]() {
+    k_fill_int<<<nblk(t.ncells + 1, 256), 256, 0, st>>>(t.ncells + 1, t.judge_fail.get());
+    k_judge_check<<<1, 256, 0, st>>>(J, t.judge_slot.get(), t.perm.get(), t.leaf_of.get(), F, t.judge_Fd.get(),
+                                     t.tol, t.judge_acc.get(), t.judge_fail.get());
+    double h[2];
+    int nf = 0;
+    FMM_HIP(hipMemcpyAsync(h, t.judge_acc.get(), sizeof(h), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipMemcpyAsync(&nf, t.judge_fail.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    jerr = h[1] > 0 ? sqrt(h[0] / h[1]) : 0.0;
+    return nf;
+  };
+  bool rerun = false;
+  if (J > 0) {
+    PhaseTimer pt(t, kTJudge, st);
+    k_judge_direct<<<J, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
+                                      t.judge_Fd.get());
+    fails = check();
+    rerun = jerr > t.tol;  // the sample's norm-wise error decides
+  }
+  if (rerun) {
+    PhaseTimer pt(t, kTJudge, st);
+    // a few failures: their leaves rerun at the full order; more than one
+    // in kJudgeDistrust of the sample: the network is not trusted on this
+    // input and every leaf reruns at the full order
+    if ((long long)fails * kJudgeDistrust > J) raise_all_leaves(t, t.order, st);
+    else raise_failing_leaves(t, t.order, st);
+    peff = std::max(std::min(orders_from_leaves(t, st), t.order), PMIN);
+    far(peff);
+    reruns = 1;
+    check();
+  }
+  t.lstats[0] = peff;
+  t.lstats[1] = fails;
+  t.lstats[2] = reruns;
+  t.lstats[3] = jerr;
+  t.lstats[4] = J;
+  t.lstats[5] = m2l_mac_fraction(t, t.order, st, &t.lstats[6]);
 }
 
 void op_batch(int op, int order, int n, const double* xyz, const double* in, int in_stride, double* out,
@@ -303,8 +510,13 @@ void direct(int n, const double* pos, const double* S, const long long* targets,
 
 // The table holds host function pointers; it exists in the host pass only.
 #if !defined(__HIP_DEVICE_COMPILE__)
-const Variant FMM_VARIANT_NAME = {FMM_VARIANT_NS::SO,   FMM_VARIANT_NS::OUT,       FMM_VARIANT_NS::PMIN,
-                                  FMM_VARIANT_NS::PMAX, &FMM_VARIANT_NS::evaluate, &FMM_VARIANT_NS::op_batch,
+const Variant FMM_VARIANT_NAME = {FMM_VARIANT_NS::SO,
+                                  FMM_VARIANT_NS::OUT,
+                                  FMM_VARIANT_NS::PMIN,
+                                  FMM_VARIANT_NS::PMAX,
+                                  &FMM_VARIANT_NS::evaluate,
+                                  &FMM_VARIANT_NS::evaluate_learned,
+                                  &FMM_VARIANT_NS::op_batch,
                                   &FMM_VARIANT_NS::direct};
 #endif
 

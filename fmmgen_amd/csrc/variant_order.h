@@ -117,28 +117,44 @@ __global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const doubl
 }
 
 template <int P>
-void ensure_index_table(Tree& t, hipStream_t st) {
-  if (t.sched.idxtab_order == P) return;
-  const std::vector<short> tab = m2l_index_table<P, SO>();
-  t.sched.idxtab.alloc(tab.size());
-  FMM_HIP(hipMemcpyAsync(t.sched.idxtab.get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
-  FMM_HIP(hipStreamSynchronize(st));
-  t.sched.idxtab_order = P;
+const short* ensure_index_table(Tree& t, hipStream_t st) {
+  M2LSchedule& S = t.sched;
+  if (!S.idx_ready[P]) {
+    const std::vector<short> tab = m2l_index_table<P, SO>();
+    S.idxtab[P].alloc(tab.size());
+    FMM_HIP(hipMemcpyAsync(S.idxtab[P].get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    S.idx_ready[P] = true;
+  }
+  return S.idxtab[P].get();
+}
+
+template <int P>
+const short* ensure_index_table32(Tree& t, hipStream_t st) {
+  M2LSchedule& S = t.sched;
+  if (!S.idx32_ready[P]) {
+    const std::vector<short> tab = m2l_index_table32<P, SO>();
+    S.idxtab32[P].alloc(tab.size());
+    FMM_HIP(hipMemcpyAsync(S.idxtab32[P].get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    S.idx32_ready[P] = true;
+  }
+  return S.idxtab32[P].get();
 }
 
 template <int P>
 void m2l_plain(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (S.ngroups == 0) return;
-  ensure_index_table<P>(t, st);
+  const short* idx = ensure_index_table<P>(t, st);
   S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
   const int grid = ((S.nchunks + 7) / 8) * 8;
   constexpr int W = m2l_waves<P, SO, double>();
   k_m2l_gemm<P, SO, double, false, W><<<grid, 64 * W, 0, st>>>(
       S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
-      S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(), S.idxtab.get(), t.group_order_ptr(), t.M.get(),
-      t.L.get());
+      S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(), idx, t.tile_order_ptr(), nullptr, Op<P>::MS,
+      Op<P>::LS, t.M.get(), t.L.get());
 }
 
 // L2L (example/calculate.cpp:219-234), x = child - parent, one thread per
@@ -216,26 +232,33 @@ void m2l_mixed(Tree& t, int mode, hipStream_t st) {
   if (S.ngroups == 0) return;
   S.Dtab32.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, float, true><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab32.get());
-  const int grid = ((S.nchunks + 7) / 8) * 8;
-  if (mode == 2) {  // BF16x3 split precision
-    if (S.idxtab32_order != P) {
-      const std::vector<short> tab = m2l_index_table32<P, SO>();
-      S.idxtab32.alloc(tab.size());
-      FMM_HIP(hipMemcpyAsync(S.idxtab32.get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
-      FMM_HIP(hipStreamSynchronize(st));
-      S.idxtab32_order = P;
-    }
-    k_m2l_bf16x3<P, SO><<<grid, 64, 0, st>>>(S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(),
-                                             S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
-                                             S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(),
-                                             S.idxtab32.get(), t.Mf.get(), t.Lf.get());
+  if (t.use_tile_order && t.m2l_classes) {  // learned order: one launch per tile-order class
+    t.m2l_classes(t, mode, P, st);
     return;
   }
-  ensure_index_table<P>(t, st);
-  constexpr int W = m2l_waves<P, SO, float>();
-  k_m2l_gemm<P, SO, float, true, W><<<grid, 64 * W, 0, st>>>(
-      S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
-      S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), S.idxtab.get(), t.group_order_ptr(), t.Mf.get(),
+  m2l_class<P>(t, mode, nullptr, S.nchunks, Op<P>::MS, Op<P>::LS, st);
+}
+
+// FP32 / BF16x3 M2L of the listed chunks (all when chunks == nullptr) at
+// order Q, on expansions and D vectors stored with row strides ms / ls.
+template <int Q>
+void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hipStream_t st) {
+  M2LSchedule& S = t.sched;
+  if (nch == 0) return;
+  const int grid = ((nch + 7) / 8) * 8;
+  if (mode == 2) {  // BF16x3 split precision
+    const short* idx = ensure_index_table32<Q>(t, st);
+    k_m2l_bf16x3<Q, SO><<<grid, 64, 0, st>>>(nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
+                                             S.chunk_g1.get(), S.group_key.get(), S.group_src.get(),
+                                             S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks,
+                                             ms, ls, t.Mf.get(), t.Lf.get());
+    return;
+  }
+  const short* idx = ensure_index_table<Q>(t, st);
+  constexpr int W = m2l_waves<Q, SO, float>();
+  k_m2l_gemm<Q, SO, float, true, W><<<grid, 64 * W, 0, st>>>(
+      nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
+      S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks, ms, ls, t.Mf.get(),
       t.Lf.get());
 }
 
@@ -319,5 +342,6 @@ void op_batch_order(int op, int n, const double* xyz, const double* in, int is, 
   template void far_field_order<P>(Tree&, int, int, hipStream_t);                                       \
   template void l2p_final_order<P>(Tree&, int, int, double*, hipStream_t);                              \
   template void op_batch_order<P>(int, int, const double*, const double*, int, double*, int, hipStream_t); \
+  template void m2l_class<P>(Tree&, int, const int*, int, int, int, hipStream_t);                         \
   }                                                                                                     \
   }

@@ -1,0 +1,266 @@
+"""Learned per-leaf expansion order: training data, training, model file.
+
+The order network (three layers, 64 hidden ReLU units, FP32 weights, FP16
+inference; kernels in csrc/mlp.hip) maps eight features of a leaf cell
+(csrc/learned.h) to the predicted far-field error of the leaf at every
+order 2..12, as log10(rms |dF| over the leaf's particles / rms |F| over all
+targets): when every leaf meets the tolerance in this sense, the
+norm-wise relative error over all targets does too.  The
+learned evaluation (mode "learned") picks per leaf the lowest order whose
+prediction plus a safety margin meets the tolerance, runs every M2L tile
+at the largest order of its target cells, and checks a sample of targets
+against an FP64 direct sum, rerunning leaves that fail at the full order.
+
+Training data are made by the plain FP64 FMM itself: one tree per
+particle set, evaluated at every order against the FP64 direct sum of all
+targets; each leaf gives one sample (its features, its 11 errors).  The
+network is trained on the GPU by the library's own forward, backward and
+Adam kernels (fmm_mlp_grad / fmm_mlp_adam).  Feature normalisation is
+folded into the first layer, so the saved parameters act on raw features.
+
+    python -m fmmgen_amd.learned --out model.npz [--samples 200000]
+"""
+import argparse
+import ctypes
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, workloads
+from .tree import build_tree, _ptr, _stream
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MODEL_PATH = os.path.join(HERE, "models", "order_mlp.npz")
+F, H, O, ORDER0 = _lib.MLP_FEATURES, _lib.MLP_HIDDEN, _lib.MLP_OUTPUTS, _lib.MLP_ORDER0
+ORDERS = list(range(ORDER0, ORDER0 + O))
+FLOOR = -16.0  # log10 of the smallest relative error kept (FP64 round-off)
+
+
+# ------------------------------------------------------------------ data
+
+def error_samples(pos, S, ncrit=64, theta=0.5, pmax=None):
+    """(X [nleaves, 8], Y [nleaves, 11]) float32 device tensors of one particle set.
+
+    Y[:, o] = log10(rms over the leaf's particles of |F_q - F_direct| /
+    rms |F_direct|) for q = ORDERS[o] (field components), from the plain
+    FP64 FMM at order q and the FP64 direct sum of every target.
+    """
+    pmax = pmax or ORDERS[-1]
+    n = pos.shape[0]
+    dpos = torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
+    dS = torch.from_numpy(np.ascontiguousarray(S, dtype=np.float64)).cuda()
+    tree = build_tree(dpos, dS, n, ncrit, pmax, theta)
+    Fd = tree.compute_field_exact(torch.empty(n, 4, dtype=torch.float64, device="cuda"))
+    fscale = torch.sqrt((Fd[:, 1:] ** 2).sum(1).mean()).item()
+    F = torch.empty_like(Fd)
+    Y = []
+    for q in ORDERS:
+        tree.set_order(q)
+        tree.compute_field_fmm(F, mode="plain")
+        e = tree.leaf_error(F, Fd) / fscale
+        Y.append(torch.log10(torch.clamp(e, min=10.0 ** FLOOR)))
+    X = tree.leaf_features()
+    return X, torch.stack(Y, 1).float()
+
+
+def problem_set(total, seed=0, nmin=1 << 16, nmax=1 << 18):
+    """Particle sets of mixed kinds and sizes until ``total`` leaves (samples) are reached."""
+    rng = np.random.default_rng(seed)
+    kinds = ["blobs", "blobs", "uniform", "plummer", "mixed", "blobs_tight", "blobs_wide"]
+    k = 0
+    while True:
+        kind = kinds[k % len(kinds)]
+        n = int(2 ** rng.uniform(np.log2(nmin), np.log2(nmax)))
+        s = int(rng.integers(1 << 30))
+        if kind == "blobs_tight":
+            pos = workloads.blobs(n, s, nblob=int(rng.integers(4, 32)), sigma=(0.003, 0.03))
+        elif kind == "blobs_wide":
+            pos = workloads.blobs(n, s, nblob=int(rng.integers(4, 32)), sigma=(0.03, 0.2),
+                                  frac_bg=float(rng.uniform(0.0, 0.5)))
+        else:
+            pos = workloads.positions(kind, n, s)
+        yield kind, pos, workloads.charges(n, s + 1)
+        k += 1
+
+
+def make_dataset(total, seed=0, log=print, **kw):
+    """(X, Y, problem id per sample) over particle sets until ``total`` samples."""
+    Xs, Ys, Ps, got = [], [], [], 0
+    t0 = time.time()
+    for k, (kind, pos, S) in enumerate(problem_set(total, seed, **kw)):
+        X, Y = error_samples(pos, S)
+        Xs.append(X)
+        Ys.append(Y)
+        Ps.append(torch.full((X.shape[0],), k, dtype=torch.int32, device=X.device))
+        got += X.shape[0]
+        log(f"  data: {kind:12s} n={pos.shape[0]:7d} leaves={X.shape[0]:6d} total={got:7d} "
+            f"({time.time() - t0:.1f}s)")
+        if got >= total:
+            break
+    return torch.cat(Xs), torch.cat(Ys), torch.cat(Ps)
+
+
+# ------------------------------------------------------------------ network
+
+def init_params(seed=0):
+    """He-uniform weights, zero biases (flat FP32 layout of the C ABI)."""
+    g = torch.Generator().manual_seed(seed)
+    parts = []
+    for nin, nout in ((F, H), (H, H), (H, O)):
+        lim = (6.0 / nin) ** 0.5
+        parts.append((torch.rand(nout, nin, generator=g) * 2 - 1) * lim)
+        parts.append(torch.zeros(nout))
+    return torch.cat([p.reshape(-1) for p in parts]).float().cuda()
+
+
+def fold_normalisation(params, mu, sigma):
+    """Parameters acting on raw features x of a network trained on (x - mu) / sigma."""
+    p = params.clone()
+    W1 = p[:H * F].view(H, F)
+    b1 = p[H * F:H * F + H]
+    W1n = W1 / sigma[None, :]
+    b1n = b1 - W1n @ mu
+    p[:H * F] = W1n.reshape(-1)
+    p[H * F:H * F + H] = b1n
+    return p
+
+
+def grad(X, T, mask, params):
+    """(loss, gradient) on the GPU (fmm_mlp_grad)."""
+    B = X.shape[0]
+    work = torch.empty(int(_lib.load().fmm_mlp_work_floats(B)), dtype=torch.float32, device="cuda")
+    g = torch.empty(_lib.MLP_PARAMS, dtype=torch.float32, device="cuda")
+    loss = torch.empty(1, dtype=torch.float32, device="cuda")
+    _lib.check(_lib.load().fmm_mlp_grad(_ptr(X), _ptr(T), _ptr(mask), B, _ptr(params), _ptr(g), _ptr(loss),
+                                        _ptr(work), _stream()))
+    return loss, g
+
+
+def forward(X, params):
+    Y = torch.empty(X.shape[0], O, dtype=torch.float32, device="cuda")
+    _lib.check(_lib.load().fmm_mlp_forward(_ptr(X), X.shape[0], _ptr(params), _ptr(Y), _stream()))
+    return Y
+
+
+def infer_f16(X, params, tol=1e-6, margin=0.0, min_order=ORDER0, max_order=ORDERS[-1]):
+    """(Y [B, 11], order [B]) from the FP16 inference kernel."""
+    B = X.shape[0]
+    Y = torch.empty(B, O, dtype=torch.float32, device="cuda")
+    q = torch.empty(B, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.load().fmm_mlp_infer_f16(_ptr(X), B, _ptr(params), _ptr(Y), _ptr(q), float(tol),
+                                             float(margin), int(min_order), int(max_order), _stream()))
+    return Y, q
+
+
+def train(X, Y, epochs=40, batch=2048, lr=3e-3, seed=0, log=print):
+    """Adam on the masked mean-square loss, on the GPU; returns raw-feature parameters."""
+    X = X.float().contiguous()
+    Y = Y.float().contiguous()
+    mu = X.mean(0)
+    sigma = X.std(0) + 1e-3
+    Xn = ((X - mu) / sigma).contiguous()
+    mask = torch.ones_like(Y)
+    params = init_params(seed)
+    m = torch.zeros_like(params)
+    v = torch.zeros_like(params)
+    B = min(batch, X.shape[0])
+    work = torch.empty(int(_lib.load().fmm_mlp_work_floats(B)), dtype=torch.float32, device="cuda")
+    g = torch.empty_like(params)
+    loss = torch.empty(1, dtype=torch.float32, device="cuda")
+    L = _lib.load()
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    nb = X.shape[0] // B
+    step = 0
+    total = epochs * nb
+    for ep in range(epochs):
+        perm = torch.randperm(X.shape[0], device="cuda", generator=gen)
+        acc = 0.0
+        for b in range(nb):
+            idx = perm[b * B:(b + 1) * B]
+            xb, yb, mb = Xn[idx].contiguous(), Y[idx].contiguous(), mask[idx].contiguous()
+            step += 1
+            lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / total))
+            _lib.check(L.fmm_mlp_grad(_ptr(xb), _ptr(yb), _ptr(mb), B, _ptr(params), _ptr(g), _ptr(loss),
+                                      _ptr(work), _stream()))
+            _lib.check(L.fmm_mlp_adam(_ptr(g), _ptr(params), _ptr(m), _ptr(v), ctypes.c_float(lr_t),
+                                      ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8), step,
+                                      _stream()))
+            if b % 16 == 0:
+                acc += loss.item()
+        if ep % 5 == 0 or ep == epochs - 1:
+            log(f"  epoch {ep:3d} loss {acc / max(1, (nb + 15) // 16):.4f}")
+    return fold_normalisation(params, mu, sigma)
+
+
+def problem_errors(Y_true, Y_pred, counts, pid, tol, margin):
+    """Norm-wise relative field error of every held-out particle set when each
+    leaf runs at its chosen order: sqrt(sum_l n_l e_l^2 / sum_l n_l), with
+    e_l the leaf's rms error relative to the set's rms field."""
+    lt = np.log10(tol)
+    ok = Y_pred + margin <= lt
+    o = np.where(ok.any(1), ok.argmax(1), O - 1)
+    e = 10.0 ** Y_true[np.arange(len(o)), o]
+    num = np.bincount(pid, weights=counts * e * e)
+    den = np.bincount(pid, weights=counts)
+    keep = den > 0
+    return np.sqrt(num[keep] / den[keep])
+
+
+def calibrate_margin(Y_true, Y_pred, X, pid, tol, target=0.5):
+    """Smallest margin (log10 units, step 0.05) for which every held-out
+    particle set meets target * tol norm-wise."""
+    yt = Y_true.cpu().numpy().astype(np.float64)
+    yp = Y_pred.cpu().numpy().astype(np.float64)
+    counts = np.exp2(X[:, 6].cpu().numpy().astype(np.float64))
+    p = pid.cpu().numpy()
+    for margin in np.arange(-1.0, 3.0, 0.05):
+        if problem_errors(yt, yp, counts, p, tol, margin).max() <= target * tol:
+            return float(margin)
+    return 3.0
+
+
+def save(path, params, margin, tol, info):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=params.detach().cpu().numpy().astype(np.float32), margin=np.float64(margin),
+             tol=np.float64(tol), info=np.array(info))
+
+
+def load(path=MODEL_PATH):
+    """(params float32 device tensor, margin, tol)."""
+    with np.load(path, allow_pickle=False) as z:
+        p = torch.from_numpy(z["params"].astype(np.float32)).cuda()
+        return p, float(z["margin"]), float(z["tol"])
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--out", default=MODEL_PATH)
+    ap.add_argument("--samples", type=int, default=200_000)
+    ap.add_argument("--holdout", type=int, default=40_000)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    t0 = time.time()
+    print("training data", flush=True)
+    X, Y, _ = make_dataset(a.samples, seed=a.seed, log=lambda s: print(s, flush=True))
+    print("held-out data", flush=True)
+    Xh, Yh, ph = make_dataset(a.holdout, seed=a.seed + 1000, log=lambda s: print(s, flush=True))
+    print(f"training on {X.shape[0]} samples", flush=True)
+    params = train(X, Y, epochs=a.epochs, seed=a.seed, log=lambda s: print(s, flush=True))
+    Yp, _ = infer_f16(Xh, params)
+    rmse = torch.sqrt(((Yp - Yh) ** 2).mean()).item()
+    margin = calibrate_margin(Yh, Yp, Xh, ph, a.tol)
+    errs = problem_errors(Yh.cpu().numpy().astype(np.float64), Yp.cpu().numpy().astype(np.float64),
+                          np.exp2(Xh[:, 6].cpu().numpy().astype(np.float64)), ph.cpu().numpy(), a.tol, margin)
+    print(f"held-out rmse {rmse:.3f} (log10 units), margin {margin:.2f}, held-out set errors "
+          f"max {errs.max():.2e} median {np.median(errs):.2e}", flush=True)
+    save(a.out, params, margin, a.tol,
+         f"samples={X.shape[0]} holdout={Xh.shape[0]} rmse={rmse:.4f} epochs={a.epochs} seed={a.seed}")
+    print(f"saved {a.out} ({time.time() - t0:.1f}s)", flush=True)
+
+
+if __name__ == "__main__":
+    main()

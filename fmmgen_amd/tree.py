@@ -63,6 +63,57 @@ class Tree:
                 pass
             self._h = None
 
+    # ----------------------------------------------------------- learned order
+    def set_order(self, order):
+        """Evaluation order (the tree and lists do not depend on it)."""
+        _lib.check(_lib.load().fmm_tree_set_order(self._h, int(order)))
+        self.order = int(order)
+        self.msize = Msize(self.order, self.source_order)
+        self.lsize = Lsize(self.order, self.source_order)
+
+    def nleaves(self):
+        return self.info()["nleaves"]
+
+    def leaves(self):
+        """Leaf cell ids in canonical leaf order (the row order of leaf data)."""
+        out = np.zeros(self.nleaves(), dtype=np.int64)
+        _lib.check(_lib.load().fmm_tree_leaves(self._h, out.ctypes.data_as(_lib.ll_p)))
+        return out
+
+    def leaf_features(self):
+        """[nleaves, 8] float32 device tensor of the leaf features (needs a
+        previous evaluation that ran P2P)."""
+        f = torch.empty(self.nleaves(), _lib.MLP_FEATURES, dtype=torch.float32, device="cuda")
+        _lib.check(_lib.load().fmm_tree_leaf_features(self._h, _ptr(f), _stream()))
+        return f
+
+    def leaf_error(self, F, Fref):
+        """[nleaves] float64 device tensor: the largest |F - Fref| (field
+        components) over each leaf's particles."""
+        _check_dev(F, "F", self.outputsize)
+        _check_dev(Fref, "Fref", self.outputsize)
+        e = torch.empty(self.nleaves(), dtype=torch.float64, device="cuda")
+        _lib.check(_lib.load().fmm_tree_leaf_error(self._h, _ptr(F), _ptr(Fref), _ptr(e), _stream()))
+        return e
+
+    def set_model(self, params, tol, margin=0.0):
+        """Order network (FP32 parameters, device tensor) of the learned modes."""
+        if params.dtype != torch.float32 or not params.is_cuda or params.numel() != _lib.MLP_PARAMS:
+            raise ValueError(f"params must be a float32 CUDA tensor of {_lib.MLP_PARAMS} values")
+        _lib.check(_lib.load().fmm_tree_set_model(self._h, _ptr(params.contiguous()), float(tol), float(margin)))
+
+    def learned_stats(self):
+        a = (ctypes.c_double * 8)()
+        _lib.check(_lib.load().fmm_tree_learned_stats(self._h, a))
+        keys = ("order_max", "judge_failures", "reruns", "judge_rel_err", "judge_targets", "m2l_mac_fraction",
+                "m2l_macs")
+        return dict(zip(keys, list(a)))
+
+    def leaf_orders(self):
+        out = np.zeros(self.nleaves(), dtype=np.int32)
+        _lib.check(_lib.load().fmm_tree_leaf_orders(self._h, out.ctypes.data_as(_lib.i_p)))
+        return out
+
     # ----------------------------------------------------------- properties
     def info(self):
         a = (ctypes.c_longlong * 16)()
@@ -77,8 +128,8 @@ class Tree:
 
     def timing_read(self):
         """{phase: (total ms, number of timed evaluations)} since timing was enabled."""
-        ms = (ctypes.c_double * 8)()
-        cnt = (ctypes.c_longlong * 8)()
+        ms = (ctypes.c_double * len(_lib.PHASE_NAMES))()
+        cnt = (ctypes.c_longlong * len(_lib.PHASE_NAMES))()
         _lib.check(_lib.load().fmm_timing_read(self._h, ms, cnt))
         return {nm: (ms[i], cnt[i]) for i, nm in enumerate(_lib.PHASE_NAMES) if nm != "unused"}
 

@@ -43,6 +43,8 @@ extern "C" {
 #define FMM_MODE_PLAIN_F64 0    /* every phase in FP64 (the reference arithmetic) */
 #define FMM_MODE_MIXED_F32 1    /* FP32 M2L on MFMA, FP32 + compensated P2P */
 #define FMM_MODE_MIXED_BF16X3 2 /* BF16x3 split M2L on MFMA, FP32 + compensated P2P */
+#define FMM_MODE_LEARNED_F32 3  /* FP32 mixed + learned per-cell order + sampled judge */
+#define FMM_MODE_LEARNED_BF16X3 4 /* as 3 with the BF16x3 M2L */
 
 /* phase mask bits for fmm_evaluate_phases */
 #define FMM_PHASE_P2M 1
@@ -99,11 +101,13 @@ int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, 
 
 /* Stream-ordered phase timing: with timing on, every evaluation records a
  * hipEvent pair around each phase on the evaluation stream.  read returns
- * the totals since the last enable in h_ms[8] / h_count[8], indexed
- * 0 prep (gather, zeroing), 1 P2M, 2 M2M, 3 M2L, 4 P2P, 5 L2L,
- * 6 L2P + final sum, 7 unused (synchronises).  No reference counterpart:
+ * the totals since the last enable in h_ms[FMM_PHASE_TIMERS] /
+ * h_count[FMM_PHASE_TIMERS], indexed 0 prep (gather, zeroing), 1 P2M,
+ * 2 M2M, 3 M2L, 4 P2P, 5 L2L, 6 L2P + final sum, 7 learned order
+ * selection, 8 judge and reruns, 9 unused (synchronises).  No reference counterpart:
  * the example times the whole evaluation with std::chrono
  * (example/main.cpp:154-162). */
+#define FMM_PHASE_TIMERS 10
 int fmm_timing_enable(fmm_tree_t tree, int enable);
 int fmm_timing_read(fmm_tree_t tree, double* h_ms, long long* h_count);
 
@@ -130,6 +134,66 @@ int fmm_export_cells(fmm_tree_t tree, double* h_geo, long long* h_meta);
 int fmm_export_particles(fmm_tree_t tree, long long* h_perm, long long* h_leaf);
 int fmm_export_lists(fmm_tree_t tree, long long* h_m2l, long long* h_p2p);
 int fmm_export_expansions(fmm_tree_t tree, double* h_M, double* h_L);
+
+/* ---------------------------------------------------------------------
+ * Learned per-cell expansion order.  No counterpart in the reference; the
+ * fixed-order evaluation above (fmm_evaluate, modes 0-2) stays the method
+ * it falls back to and the judge it is checked against.
+ *
+ * Network: FMM_MLP_FEATURES inputs, two hidden ReLU layers of
+ * FMM_MLP_HIDDEN units, FMM_MLP_OUTPUTS outputs; FP32 parameters, flat
+ * [W1 (H x F), b1 (H), W2 (H x H), b2 (H), W3 (O x H), b3 (O)], row-major.
+ * Output o predicts log10(rms |dF| over a leaf's particles / rms |F| over
+ * all targets) of the far field at order FMM_MLP_ORDER0 + o.  All pointers are device
+ * pointers except where named h_. */
+#define FMM_MLP_FEATURES 8
+#define FMM_MLP_HIDDEN 64
+#define FMM_MLP_OUTPUTS 11
+#define FMM_MLP_ORDER0 2
+#define FMM_MLP_PARAMS 5451
+
+/* FP32 forward: d_Y [B*O]. */
+int fmm_mlp_forward(const float* d_X, int B, const float* d_params, float* d_Y, void* stream);
+/* FP16 inference (FP32 accumulation): d_Y [B*O] (NULL skips) and d_order [B]
+ * (NULL skips) = the lowest order FMM_MLP_ORDER0 + o with
+ * Y[o] + margin <= log10(tol), else max_order, clamped to
+ * [min_order, max_order]. */
+int fmm_mlp_infer_f16(const float* d_X, int B, const float* d_params, float* d_Y, int* d_order, double tol,
+                      double margin, int min_order, int max_order, void* stream);
+/* Masked mean-square loss over the outputs with d_mask != 0 and its
+ * gradient: d_grad [FMM_MLP_PARAMS], d_loss [1]; d_work holds
+ * fmm_mlp_work_floats(B) floats. */
+size_t fmm_mlp_work_floats(int B);
+int fmm_mlp_grad(const float* d_X, const float* d_T, const float* d_mask, int B, const float* d_params,
+                 float* d_grad, float* d_loss, float* d_work, void* stream);
+/* One bias-corrected Adam step (step >= 1) on the parameters. */
+int fmm_mlp_adam(const float* d_grad, float* d_params, float* d_m, float* d_v, float lr, float beta1, float beta2,
+                 float eps, int step, void* stream);
+
+/* Evaluation order of the tree (any compiled order: the tree and lists
+ * do not depend on it); the learned modes use it as the full order. */
+int fmm_tree_set_order(fmm_tree_t tree, int order);
+/* Leaf features d_feat [nleaves*FMM_MLP_FEATURES], leaves in canonical
+ * order; they use the near field of the last evaluation that ran P2P. */
+int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream);
+/* Root-mean-square |F - Fref| (field components) over each leaf's
+ * particles: d_err [nleaves]; d_F, d_Fref [n*OUTPUTSIZE] in caller order. */
+int fmm_tree_leaf_error(fmm_tree_t tree, const double* d_F, const double* d_Fref, double* d_err, void* stream);
+/* Leaf cell ids in canonical leaf order: h_leaves [nleaves]. */
+int fmm_tree_leaves(fmm_tree_t tree, long long* h_leaves);
+/* Network (copied, converted to FP16), tolerance and safety margin (in
+ * log10 units) of the learned modes. */
+int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, double margin);
+/* Statistics of the last learned evaluation, h_stats [8]: largest order
+ * used, judge targets above tol * rms |F| (a rerun happens when the
+ * sample's rel. field error exceeds tol), reruns (0/1), judge rel. field
+ * error (after the rerun, if any),
+ * judge targets, M2L multiply-adds relative to the tree's order, M2L
+ * useful multiply-adds (sum over pairs of C(q + 6, 6) at the pair's
+ * order), 0. */
+int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats);
+/* Per-leaf orders of the last learned evaluation: h_order [nleaves]. */
+int fmm_tree_leaf_orders(fmm_tree_t tree, int* h_order);
 
 #ifdef __cplusplus
 }
