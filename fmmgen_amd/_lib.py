@@ -19,7 +19,7 @@ MODE_LEARNED_BF16X3 = 4
 MODES = {"plain": MODE_PLAIN_F64, "f32": MODE_MIXED_F32, "bf16x3": MODE_MIXED_BF16X3,
          "learned": MODE_LEARNED_F32, "learned_bf16x3": MODE_LEARNED_BF16X3}
 
-MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_ORDER0, MLP_PARAMS = 8, 64, 11, 2, 5451
+MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_ORDER0, MLP_PARAMS = 12, 64, 11, 2, 5707
 
 PHASE_P2M, PHASE_M2M, PHASE_M2L, PHASE_P2P, PHASE_L2L, PHASE_L2P = 1, 2, 4, 8, 16, 32
 PHASE_ALL = 63
@@ -34,7 +34,7 @@ EXPORTS = (
     "fmm_timing_enable", "fmm_timing_read",
     "fmm_mlp_forward", "fmm_mlp_infer_f16", "fmm_mlp_work_floats", "fmm_mlp_grad", "fmm_mlp_adam",
     "fmm_tree_set_order", "fmm_tree_leaf_features", "fmm_tree_leaf_error", "fmm_tree_leaves",
-    "fmm_tree_set_model", "fmm_tree_learned_stats", "fmm_tree_leaf_orders",
+    "fmm_tree_set_model", "fmm_tree_learned_stats", "fmm_tree_leaf_orders", "fmm_tree_near_scale",
 )
 PHASE_NAMES = ("prep", "P2M", "M2M", "M2L", "P2P", "L2L", "L2P", "order", "judge", "unused")
 
@@ -100,6 +100,7 @@ def load(path=None):
     L.fmm_tree_set_model.argtypes = [vp, vp, cd, cd]
     L.fmm_tree_learned_stats.argtypes = [vp, d_p]
     L.fmm_tree_leaf_orders.argtypes = [vp, i_p]
+    L.fmm_tree_near_scale.argtypes = [vp, d_p]
     for nm in EXPORTS:
         if nm not in ("fmm_last_error",):
             getattr(L, nm).restype = ci

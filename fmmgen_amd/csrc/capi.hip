@@ -402,6 +402,13 @@ int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream) {
   });
 }
 
+int fmm_tree_near_scale(fmm_tree_t tree, double* h_scale) {
+  return guarded([&] {
+    FMM_CHECK(tree && h_scale, kInvalidArgument, "NULL argument");
+    *h_scale = tree->t.near_scale;
+  });
+}
+
 int fmm_tree_leaf_error(fmm_tree_t tree, const double* d_F, const double* d_Fref, double* d_err, void* stream) {
   return guarded([&] {
     FMM_CHECK(tree && d_F && d_Fref && d_err, kInvalidArgument, "NULL argument");
@@ -438,7 +445,7 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
 int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats) {
   return guarded([&] {
     FMM_CHECK(tree && h_stats, kInvalidArgument, "NULL argument");
-    for (int k = 0; k < 8; ++k) h_stats[k] = tree->t.lstats[k];
+    for (int k = 0; k < 10; ++k) h_stats[k] = tree->t.lstats[k];
   });
 }
 

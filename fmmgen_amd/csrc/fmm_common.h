@@ -121,6 +121,8 @@ struct M2LSchedule {
   // learned order: chunks sorted by their tile's order, and the start of
   // every order's range (kMaxOrder + 1 entries)
   DevBuf<int> class_chunks;
+  DevBuf<long long> tile_pairs;  // M2L pairs per tile (statistics)
+  bool tile_pairs_ready = false;
   // work items: chunks of at most kGroupsPerChunk groups of one tile
   int nchunks = 0;
   DevBuf<int> chunk_tile, chunk_g0, chunk_g1;
@@ -178,9 +180,10 @@ struct Tree {
   // order of its 16 target cells; the tile's groups are truncated there)
   DevBuf<float> leaf_feat;         // [nleaves * kMlpF]
   DevBuf<int> leaf_q, cell_q;      // [nleaves], [ncells]
-  DevBuf<double> cell_qabs;        // [ncells] sum of |source strength| below the cell
-  DevBuf<double> cell_acc;         // [ncells * 7] list sums, accumulated down the tree
-  DevBuf<double> fscale;           // [2] sum of squares of the near field, rms
+  DevBuf<float> leaf_y;            // [nleaves * kMlpO] network outputs
+  DevBuf<double> cell_qabs, cell_q2;  // [ncells] sum of |S| and of |S|^2 below the cell
+  DevBuf<double> cell_acc;            // [ncells * 11] list sums, accumulated down the tree
+  double near_scale = 0.0;         // F_near of the last feature pass
   DevBuf<signed char> tile_order;  // [ntiles]
   bool use_tile_order = false;
   const signed char* tile_order_ptr() const { return use_tile_order ? tile_order.get() : nullptr; }
@@ -195,7 +198,7 @@ struct Tree {
   DevBuf<int> judge_slot;
   DevBuf<double> judge_Fd, judge_acc;  // [judge_n * OUT], [4]
   DevBuf<int> judge_fail;              // [1 + ncells] failing count, per cell flag
-  double lstats[8] = {0};
+  double lstats[10] = {0};
 
   // expansions, [ncells * Msize] and [ncells * Lsize] in FP64
   DevBuf<double> M, L;

@@ -1,16 +1,23 @@
 // Learned per-cell expansion order (no counterpart in the reference).
 //
-// Per leaf, eight features describe how strongly the far field reaches the
-// leaf: with rho = (rmax_A + rmax_B) / R and w = sum|S|_B / R^2 over every
-// M2L pair (B -> A) of the leaf and of its ancestors,
-//   f0..f3  log10(sum w rho^k / F_near), k = 3, 6, 9, 12
-//   f4      largest rho
-//   f5      log10(1 + number of those pairs)
-//   f6      log2(particles in the leaf)
-//   f7      log10(sum w / F_near)
-// where F_near is the rms near-field (P2P) magnitude of the evaluation.  The
+// Per leaf, twelve features describe how strongly the far field reaches
+// the leaf: with rho = (rmax_A + rmax_B) / R, w = sum|S|_B / R^2 and
+// v = sqrt(sum |S|^2)_B / R^2 over every M2L pair (B -> A) of the leaf and
+// of its ancestors (v is the size of a multipole of random-sign sources),
+//   f0..f3   log10(sum w rho^k / F_near), k = 3, 6, 9, 12
+//   f4       largest rho
+//   f5       log10(1 + number of those pairs)
+//   f6       log2(particles in the leaf)
+//   f7       log10(sum w / F_near)
+//   f8..f11  log10(sum v rho^k / F_near), k = 3, 6, 9, 12
+// where F_near is the rms near-field (P2P) magnitude of the evaluation over
+// the targets outside the largest 0.1% of magnitudes (the field of nearly
+// coincident pairs would dominate a plain rms, and a sample of targets
+// seldom holds one).  The
 // network maps them to the predicted rms far-field error of the leaf per
-// order (relative to the rms field of all targets); a cell's
+// order (relative to F_near); the leaf orders are
+// the cheapest set whose predicted norm-wise error over all targets meets
+// the tolerance; a cell's
 // order is the largest order of the leaves below it, and an M2L tile runs
 // at the largest order of its target cells.
 #pragma once
@@ -26,6 +33,12 @@ void leaf_features(Tree& t, int ss, int outsz, int fo, hipStream_t st);
 // Root-mean-square |F - Fref| (field components) over the particles of
 // each leaf; F and Fref in caller order.
 void leaf_error(Tree& t, const double* F, const double* Fref, int outsz, int fo, double* err, hipStream_t st);
+// Leaf orders into t.leaf_q from the network outputs Y [nleaves][O]: the
+// cheapest orders (M2L multiply-adds) whose predicted norm-wise field
+// error relative to F_near, sqrt(sum_l n_l e_l^2 / n) with
+// e_l = 10^(Y_l + margin), meets tol (a Lagrange multiplier found by
+// bisection); returns that predicted error.
+double allocate_orders(Tree& t, const float* Y, double tol, double margin, int qmin, int qmax, hipStream_t st);
 // Cell and tile orders from t.leaf_q; returns the largest order (the
 // root's), read back to the host.
 int orders_from_leaves(Tree& t, hipStream_t st);

@@ -10,7 +10,7 @@ namespace fmm {
 namespace {
 
 inline int nb(long long n, int bs) { return (int)((n + bs - 1) / bs); }
-constexpr int kAcc = 7;  // per-cell list sums
+constexpr int kAcc = 11;  // per-cell list sums
 
 __global__ void k_zero_d(long long n, double* __restrict__ a) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -22,20 +22,22 @@ __global__ void k_fill_i(long long n, int* __restrict__ a, int v) {
   if (i < n) a[i] = v;
 }
 
-// sum |S| over the particles of each leaf
+// sum |S| and sum |S|^2 over the particles of each leaf
 __global__ void k_leaf_qabs(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pstart,
                             const int* __restrict__ pcount, const double* __restrict__ pS, int ss,
-                            double* __restrict__ qabs) {
+                            double* __restrict__ qabs, double* __restrict__ q2) {
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= nleaves) return;
   const int c = leaves[li];
-  double a = 0.0;
+  double a = 0.0, b = 0.0;
   for (int j = pstart[c]; j < pstart[c] + pcount[c]; ++j) {
     double s2 = 0.0;
     for (int k = 0; k < ss; ++k) s2 += pS[(size_t)j * ss + k] * pS[(size_t)j * ss + k];
     a += sqrt(s2);
+    b += s2;
   }
   qabs[c] = a;
+  q2[c] = b;
 }
 
 // parents of one level: sum of the children (children are contiguous)
@@ -57,37 +59,51 @@ __global__ void k_parent_max(int beg, int end, const int* __restrict__ first_chi
   q[c] = a;
 }
 
-// list sums of every target cell: acc[c] = (S0, S3, S6, S9, S12, max rho, count)
-__global__ void k_cell_lists(int ncells, const int* __restrict__ off, const int* __restrict__ src,
-                             const double* __restrict__ cx, const double* __restrict__ cy,
-                             const double* __restrict__ cz, const double* __restrict__ crmax,
-                             const double* __restrict__ qabs, double* __restrict__ acc) {
-  const int A = blockIdx.x * blockDim.x + threadIdx.x;
+// list sums of every target cell: acc[c] = (S0, S3, S6, S9, S12 with
+// w = sum|S| / R^2; T3, T6, T9, T12 with w = sqrt(sum |S|^2) / R^2; max rho;
+// count), S_k = sum w rho^k; one wavefront per cell, lanes over its list
+__global__ __launch_bounds__(64) void k_cell_lists(int ncells, const int* __restrict__ off,
+                                                   const int* __restrict__ src, const double* __restrict__ cx,
+                                                   const double* __restrict__ cy, const double* __restrict__ cz,
+                                                   const double* __restrict__ crmax, const double* __restrict__ qabs,
+                                                   const double* __restrict__ q2, double* __restrict__ acc) {
+  const int A = blockIdx.x;
   if (A >= ncells) return;
-  double s0 = 0, s3 = 0, s6 = 0, s9 = 0, s12 = 0, rmx = 0;
-  const double xa = cx[A], ya = cy[A], za = cz[A], ra = crmax[A];
-  for (int i = off[A]; i < off[A + 1]; ++i) {
+  // features are estimates: per-pair arithmetic in FP32 (offsets from
+  // FP64 differences), sums in FP64
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  float rmx = 0.f;
+  const double xa = cx[A], ya = cy[A], za = cz[A];
+  const float ra = (float)crmax[A];
+  for (int i = off[A] + threadIdx.x; i < off[A + 1]; i += 64) {
     const int B = src[i];
-    const double dx = xa - cx[B], dy = ya - cy[B], dz = za - cz[B];
-    const double R2 = dx * dx + dy * dy + dz * dz;
-    const double rho = (ra + crmax[B]) * rsqrt(R2);
-    const double w = qabs[B] / R2;
-    const double r3 = rho * rho * rho, r6 = r3 * r3;
-    s0 += w;
-    s3 += w * r3;
-    s6 += w * r6;
-    s9 += w * r6 * r3;
-    s12 += w * r6 * r6;
-    rmx = fmax(rmx, rho);
+    const float dx = (float)(xa - cx[B]), dy = (float)(ya - cy[B]), dz = (float)(za - cz[B]);
+    const float R2 = dx * dx + dy * dy + dz * dz;
+    const float rho = (ra + (float)crmax[B]) * __frsqrt_rn(R2);
+    const float w = (float)qabs[B] / R2, u = sqrtf((float)q2[B]) / R2;
+    const float r3 = rho * rho * rho, r6 = r3 * r3, r9 = r6 * r3, r12 = r6 * r6;
+    v[0] += w;
+    v[1] += w * r3;
+    v[2] += w * r6;
+    v[3] += w * r9;
+    v[4] += w * r12;
+    v[5] += u * r3;
+    v[6] += u * r6;
+    v[7] += u * r9;
+    v[8] += u * r12;
+    rmx = fmaxf(rmx, rho);
   }
-  double* a = acc + (size_t)A * kAcc;
-  a[0] = s0;
-  a[1] = s3;
-  a[2] = s6;
-  a[3] = s9;
-  a[4] = s12;
-  a[5] = rmx;
-  a[6] = (double)(off[A + 1] - off[A]);
+  for (int m = 32; m > 0; m >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) v[k] += __shfl_xor(v[k], m, 64);
+    rmx = fmaxf(rmx, __shfl_xor(rmx, m, 64));
+  }
+  if (threadIdx.x == 0) {
+    double* a = acc + (size_t)A * kAcc;
+    for (int k = 0; k < 9; ++k) a[k] = v[k];
+    a[9] = rmx;
+    a[10] = (double)(off[A + 1] - off[A]);
+  }
 }
 
 // cells of one level add their parent's accumulated sums (root first)
@@ -96,41 +112,55 @@ __global__ void k_acc_down(int beg, int end, const int* __restrict__ parent, dou
   if (c >= end) return;
   const double* p = acc + (size_t)parent[c] * kAcc;
   double* a = acc + (size_t)c * kAcc;
-  for (int k = 0; k < 5; ++k) a[k] += p[k];
-  a[5] = fmax(a[5], p[5]);
-  a[6] += p[6];
+  for (int k = 0; k < 9; ++k) a[k] += p[k];
+  a[9] = fmax(a[9], p[9]);
+  a[10] += p[10];
 }
 
-// sum of squares of the field components of the sorted near field
-__global__ void k_sumsq(long long n, int outsz, int fo, const double* __restrict__ F, double* __restrict__ out) {
-  __shared__ double red[256];
-  double s = 0.0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    for (int k = fo; k < outsz; ++k) s += F[i * outsz + k] * F[i * outsz + k];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+// Histogram of log2 |F| (field components) of the sorted near field:
+// kHistBins bins of width 1/4 from 2^kHistLo, counts and sums of |F|^2.
+constexpr int kHistBins = 512;
+constexpr double kHistLo = -64.0;
+
+__global__ void k_field_hist(long long n, int outsz, int fo, const double* __restrict__ F,
+                             unsigned long long* __restrict__ cnt, double* __restrict__ s2) {
+  __shared__ unsigned int lc[kHistBins];
+  __shared__ float ls[kHistBins];
+  for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) {
+    lc[b] = 0u;
+    ls[b] = 0.f;
   }
-  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+  __syncthreads();
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double m2 = 0.0;
+    for (int k = fo; k < outsz; ++k) m2 += F[i * outsz + k] * F[i * outsz + k];
+    int b = m2 > 0.0 ? (int)floor((0.5 * log2(m2) - kHistLo) * 4.0) : 0;
+    b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
+    atomicAdd(&lc[b], 1u);
+    atomicAdd(&ls[b], (float)(m2 * exp2(-2.0 * (kHistLo + 0.25 * b))));  // scaled into [1, 2^0.5)^2
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kHistBins; b += blockDim.x)
+    if (lc[b]) {
+      atomicAdd(&cnt[b], (unsigned long long)lc[b]);
+      atomicAdd(&s2[b], (double)ls[b] * exp2(2.0 * (kHistLo + 0.25 * b)));
+    }
 }
 
 __global__ void k_leaf_feat(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pcount,
-                            const double* __restrict__ acc,
-                            const double* __restrict__ fscale, long long n, float* __restrict__ feat) {
+                            const double* __restrict__ acc, double fn, float* __restrict__ feat) {
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= nleaves) return;
   const int c = leaves[li];
-  const double fn = sqrt(fscale[0] / (double)(n > 0 ? n : 1));
   const double* a = acc + (size_t)c * kAcc;
   float* f = feat + (size_t)li * kMlpF;
   auto lg = [&](double v) { return (float)fmin(fmax(log10(fmax(v, 1e-300) / fmax(fn, 1e-300)), -40.0), 20.0); };
   for (int k = 0; k < 4; ++k) f[k] = lg(a[1 + k]);
-  f[4] = (float)a[5];
-  f[5] = (float)log10(1.0 + a[6]);
+  f[4] = (float)a[9];
+  f[5] = (float)log10(1.0 + a[10]);
   f[6] = (float)log2((double)max(pcount[c], 1));
   f[7] = lg(a[0]);
+  for (int k = 0; k < 4; ++k) f[8 + k] = lg(a[5 + k]);
 }
 
 __global__ void k_leaf_err(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pstart,
@@ -180,16 +210,49 @@ __device__ __forceinline__ double macs(int q) {  // C(q + 6, 6)
   return a;
 }
 
-// pairs of each tile (valid group columns) times C(q + 6, 6)
-__global__ void k_mac_frac(int ntiles, const int* __restrict__ goff, const int* __restrict__ gsrc,
-                           const signed char* __restrict__ to, int pref, double* __restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// M2L pairs of every tile (valid columns of its groups), one block per tile
+__global__ void k_tile_pairs(int ntiles, const int* __restrict__ goff, const int* __restrict__ gsrc,
+                             long long* __restrict__ pairs) {
+  __shared__ long long red[256];
+  const int t = blockIdx.x;
   if (t >= ntiles) return;
-  long long pairs = 0;
-  for (long long i = (long long)goff[t] * kTile; i < (long long)goff[t + 1] * kTile; ++i) pairs += gsrc[i] >= 0;
-  const int q = to ? (int)to[t] : pref;
-  atomicAdd(&out[0], (double)pairs * macs(q));
-  atomicAdd(&out[1], (double)pairs * macs(pref));
+  long long c = 0;
+  for (long long i = (long long)goff[t] * kTile + threadIdx.x; i < (long long)goff[t + 1] * kTile; i += blockDim.x)
+    c += gsrc[i] >= 0;
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pairs[t] = red[0];
+}
+
+// pairs of each tile times C(q + 6, 6)
+__global__ void k_mac_frac(int ntiles, const long long* __restrict__ pairs, const signed char* __restrict__ to,
+                           int pref, double* __restrict__ out) {
+  __shared__ double r0[256], r1[256];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (t < ntiles) {
+    const int q = to ? (int)to[t] : pref;
+    a = (double)pairs[t] * macs(q);
+    b = (double)pairs[t] * macs(pref);
+  }
+  r0[threadIdx.x] = a;
+  r1[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r0[threadIdx.x] += r0[threadIdx.x + s];
+      r1[threadIdx.x] += r1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], r0[0]);
+    atomicAdd(&out[1], r1[0]);
+  }
 }
 
 __global__ void k_class_keys(int nch, const int* __restrict__ chunk_tile, const signed char* __restrict__ to,
@@ -207,7 +270,75 @@ __global__ void k_class_split(int nch, const unsigned long long* __restrict__ ke
   if (i == 0 || (int)(key[i - 1] >> 32) != q) start[q] = i;
 }
 
+// Orders of the leaves for one Lagrange multiplier lam (10^loglam): leaf l
+// takes the order q minimising C(q + 6, 6) w_l + lam n_l e_l(q)^2, with
+// e_l(q) = 10^(Y_l(q) + margin) its predicted rms error relative to the rms
+// field, n_l its particles (2^f6) and w_l the M2L pairs on its ancestor
+// chain (10^f5 - 1), the pairs whose order it raises.  Adds sum n_l e_l^2.
+__global__ void k_alloc(int nl, const float* __restrict__ feat, const float* __restrict__ Y, double loglam,
+                        float margin, int qmin, int qmax, int* __restrict__ q_out, double* __restrict__ err_sum) {
+  __shared__ double red[256];
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  double contrib = 0.0;
+  if (li < nl) {
+    const float* f = feat + (size_t)li * kMlpF;
+    const double w = fmax(exp10((double)f[5]) - 1.0, 1.0);
+    const double n = exp2((double)f[6]);
+    const double lam = exp10(loglam);
+    double best = 0.0, be = 0.0;
+    int bq = -1;
+    for (int o = 0; o < kMlpO; ++o) {
+      const int q = o + kMlpOrder0;
+      if (q < qmin || q > qmax) continue;
+      const double e2 = exp10(2.0 * ((double)Y[(size_t)li * kMlpO + o] + margin));
+      const double c = macs(q) * w + lam * n * e2;
+      if (bq < 0 || c < best) {
+        best = c;
+        bq = q;
+        be = e2;
+      }
+    }
+    if (bq < 0) bq = qmax;
+    q_out[li] = bq;
+    contrib = n * be;
+  }
+  red[threadIdx.x] = contrib;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(err_sum, red[0]);
+}
+
 }  // namespace
+
+double allocate_orders(Tree& t, const float* Y, double tol, double margin, int qmin, int qmax, hipStream_t st) {
+  const int nl = t.nleaves;
+  if (nl == 0) return 0.0;
+  DevBuf<double> acc;
+  acc.alloc(1);
+  const double target = tol * tol * (double)t.n;
+  auto eval = [&](double loglam) {
+    k_zero_d<<<1, 64, 0, st>>>(1, acc.get());
+    k_alloc<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, loglam, (float)margin, qmin, qmax,
+                                         t.leaf_q.get(), acc.get());
+    double h = 0.0;
+    FMM_HIP(hipMemcpyAsync(&h, acc.get(), sizeof(double), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    return h;
+  };
+  double lo = -10.0, hi = 40.0;
+  double e = eval(hi);
+  if (e > target) return sqrt(e / (double)t.n);  // even the highest orders miss: best effort
+  for (int it = 0; it < 34; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (eval(mid) <= target) hi = mid;
+    else lo = mid;
+  }
+  e = eval(hi);
+  return sqrt(e / (double)t.n);
+}
 
 void chunk_classes(Tree& t, hipStream_t st, int* start) {
   constexpr int NQ = M2LSchedule::kMaxOrder;
@@ -242,27 +373,53 @@ void chunk_classes(Tree& t, hipStream_t st, int* start) {
 void leaf_features(Tree& t, int ss, int outsz, int fo, hipStream_t st) {
   const int nc = t.ncells, nl = t.nleaves;
   t.cell_qabs.alloc(nc);
+  t.cell_q2.alloc(nc);
   t.cell_acc.alloc((size_t)nc * kAcc);
-  t.fscale.alloc(2);
   t.leaf_feat.alloc((size_t)std::max(nl, 1) * kMlpF);
   k_zero_d<<<nb(nc, 256), 256, 0, st>>>(nc, t.cell_qabs.get());
-  k_zero_d<<<1, 64, 0, st>>>(2, t.fscale.get());
+  k_zero_d<<<nb(nc, 256), 256, 0, st>>>(nc, t.cell_q2.get());
   if (nl == 0) return;
   k_leaf_qabs<<<nb(nl, 128), 128, 0, st>>>(nl, t.leaves.get(), t.pstart.get(), t.pcount.get(), t.pS.get(), ss,
-                                          t.cell_qabs.get());
+                                          t.cell_qabs.get(), t.cell_q2.get());
   for (int l = t.depth - 2; l >= 0; --l) {
     const int b = t.level_off[l], e = t.level_off[l + 1];
     k_parent_sum<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cell_qabs.get());
+    k_parent_sum<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cell_q2.get());
   }
-  k_cell_lists<<<nb(nc, 128), 128, 0, st>>>(nc, t.m2l_off.get(), t.m2l_src.get(), t.cx.get(), t.cy.get(), t.cz.get(),
-                                           t.crmax.get(), t.cell_qabs.get(), t.cell_acc.get());
+  k_cell_lists<<<nc, 64, 0, st>>>(nc, t.m2l_off.get(), t.m2l_src.get(), t.cx.get(), t.cy.get(), t.cz.get(),
+                                           t.crmax.get(), t.cell_qabs.get(), t.cell_q2.get(), t.cell_acc.get());
   for (int l = 1; l < t.depth; ++l) {
     const int b = t.level_off[l], e = t.level_off[l + 1];
     k_acc_down<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.parent.get(), t.cell_acc.get());
   }
+  // F_near: rms |F| of the near field over the targets below the top
+  // 0.1% of magnitudes (near-coincident pairs would dominate a plain rms)
   const long long n = (long long)t.n;
-  k_sumsq<<<std::min(nb(n, 256), 2048), 256, 0, st>>>(n, outsz, fo, t.Fp2p.get(), t.fscale.get());
-  k_leaf_feat<<<nb(nl, 128), 128, 0, st>>>(nl, t.leaves.get(), t.pcount.get(), t.cell_acc.get(), t.fscale.get(), n,
+  DevBuf<unsigned long long> hc;
+  DevBuf<double> hs;
+  hc.alloc(kHistBins);
+  hs.alloc(kHistBins);
+  k_zero_d<<<nb(kHistBins, 256), 256, 0, st>>>(kHistBins, reinterpret_cast<double*>(hc.get()));
+  k_zero_d<<<nb(kHistBins, 256), 256, 0, st>>>(kHistBins, hs.get());
+  k_field_hist<<<std::min(nb(n, 256), 1024), 256, 0, st>>>(n, outsz, fo, t.Fp2p.get(), hc.get(), hs.get());
+  std::vector<unsigned long long> c(kHistBins);
+  std::vector<double> s2(kHistBins);
+  FMM_HIP(hipMemcpyAsync(c.data(), hc.get(), sizeof(unsigned long long) * kHistBins, hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipMemcpyAsync(s2.data(), hs.get(), sizeof(double) * kHistBins, hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  const unsigned long long drop = (unsigned long long)(0.001 * (double)n);
+  unsigned long long above = 0, kept = 0;
+  double sum = 0.0;
+  for (int b = kHistBins - 1; b >= 0; --b) {
+    if (above + c[b] <= drop) {
+      above += c[b];
+      continue;
+    }
+    kept += c[b];
+    sum += s2[b];
+  }
+  t.near_scale = kept > 0 ? sqrt(sum / (double)kept) : 1.0;
+  k_leaf_feat<<<nb(nl, 128), 128, 0, st>>>(nl, t.leaves.get(), t.pcount.get(), t.cell_acc.get(), t.near_scale,
                                           t.leaf_feat.get());
 }
 
@@ -310,8 +467,13 @@ double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out) {
   DevBuf<double> acc;
   acc.alloc(2);
   k_zero_d<<<1, 64, 0, st>>>(2, acc.get());
-  k_mac_frac<<<nb(nt, 64), 64, 0, st>>>(nt, t.sched.tile_group_off.get(), t.sched.group_src.get(),
-                                        t.tile_order_ptr(), pref, acc.get());
+  if (!t.sched.tile_pairs_ready) {  // order independent: once per tree
+    t.sched.tile_pairs.alloc(nt);
+    k_tile_pairs<<<nt, 256, 0, st>>>(nt, t.sched.tile_group_off.get(), t.sched.group_src.get(),
+                                     t.sched.tile_pairs.get());
+    t.sched.tile_pairs_ready = true;
+  }
+  k_mac_frac<<<nb(nt, 256), 256, 0, st>>>(nt, t.sched.tile_pairs.get(), t.tile_order_ptr(), pref, acc.get());
   double h[2] = {0, 0};
   FMM_HIP(hipMemcpyAsync(h, acc.get(), sizeof(h), hipMemcpyDeviceToHost, st));
   FMM_HIP(hipStreamSynchronize(st));

@@ -14,7 +14,7 @@
 
 namespace fmm {
 
-constexpr int kMlpF = 8;
+constexpr int kMlpF = 12;
 constexpr int kMlpH = 64;
 constexpr int kMlpO = 11;
 constexpr int kMlpOrder0 = 2;  // order of output 0 (orders 2 .. 12)

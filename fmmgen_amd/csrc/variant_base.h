@@ -2,6 +2,8 @@
 // near field, the FP64 direct sum, order dispatch and the variant table.
 #pragma once
 
+#include <algorithm>
+
 #include "learned.h"
 #include "variant_decl.h"
 
@@ -61,7 +63,10 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
 }
 
 // Sampled judge of the learned order: FP64 direct sum (self excluded) at
-// sorted slots, one block per target.
+// sorted slots; block (t, b) sums the sources of slice b of target t and
+// adds its partial sum atomically into the zeroed Fd.
+constexpr int kJudgeSlice = 16384;
+
 __global__ void k_judge_direct(int n, const double* __restrict__ px, const double* __restrict__ py,
                                const double* __restrict__ pz, const double* __restrict__ pS,
                                const int* __restrict__ slots, int J, double* __restrict__ Fd) {
@@ -72,7 +77,8 @@ __global__ void k_judge_direct(int n, const double* __restrict__ px, const doubl
   double acc[OUT];
 #pragma unroll
   for (int k = 0; k < OUT; ++k) acc[k] = 0.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  const int b = blockIdx.y * kJudgeSlice, e = min(n, b + kJudgeSlice);
+  for (int j = b + threadIdx.x; j < e; j += blockDim.x) {
     if (j == i) continue;
     fmmgen_ops::P2P<double>(xi - px[j], yi - py[j], zi - pz[j], pS + (size_t)j * SS, acc);
   }
@@ -88,7 +94,7 @@ __global__ void k_judge_direct(int n, const double* __restrict__ px, const doubl
   }
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < OUT; ++k) Fd[(size_t)t * OUT + k] = red[0][k];
+    for (int k = 0; k < OUT; ++k) atomicAdd(&Fd[(size_t)t * OUT + k], red[0][k]);
 }
 
 // Judge check (one block): a target fails when |F - Fd| over the field
@@ -359,13 +365,15 @@ void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
 }
 
 // Learned-order evaluation: FP32 P2P, leaf features, FP16 network
-// inference -> leaf orders -> cell and tile orders, mixed-precision far
+// inference -> leaf orders (the cheapest whose predicted norm-wise error
+// meets the tolerance) -> cell and tile orders, mixed-precision far
 // field at the largest order with every M2L tile truncated at its own
 // order, then the sampled judge: an FP64 direct sum at kJudgeTargets
 // targets.  When the sample's relative field error exceeds the tolerance,
 // the leaves holding a target whose error exceeds tol * rms |F| rerun at
-// the tree's order (all leaves when more than one target in
-// kJudgeDistrust does).
+// the tree's order and the other leaves are re-allocated for half the
+// tolerance (all leaves run at the tree's order when more than one target
+// in kJudgeDistrust fails).
 // mode 3: FP32 M2L, mode 4: BF16x3 M2L.
 // M2L of the learned evaluation: chunks grouped by their tile's order,
 // each group launched with the kernel of its own order on the expansions
@@ -380,7 +388,7 @@ void m2l_by_class(Tree& t, int mode, int P, hipStream_t st) {
   }
 }
 
-constexpr int kJudgeTargets = 256;
+constexpr int kJudgeTargets = 1024;
 constexpr int kJudgeDistrust = 64;
 
 void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
@@ -388,6 +396,7 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   constexpr int FO = OUT == 4 ? 1 : 0;
   const int n = (int)t.n, nl = t.nleaves;
   const int fmode = mode == 4 ? 2 : 1;
+  const int qmin = std::max(PMIN, kMlpOrder0);
   t.Fp2p.alloc((size_t)std::max(n, 1) * OUT);
   t.pS.alloc((size_t)std::max(n, 1) * SS);
   {
@@ -395,13 +404,56 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
     k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
   }
   near_field(t, 1, st);
+  // judge sample: FP64 direct sums first; their field scale sets the
+  // tolerance the orders are allocated for
+  const int J = std::min(kJudgeTargets, n);
+  if (t.judge_n != J) {
+    std::vector<int> h(J);
+    for (int i = 0; i < J; ++i) h[i] = (int)(((long long)(2 * i + 1) * n) / (2LL * J));
+    t.judge_slot.alloc(std::max(J, 1));
+    FMM_HIP(hipMemcpyAsync(t.judge_slot.get(), h.data(), sizeof(int) * J, hipMemcpyHostToDevice, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    t.judge_n = J;
+  }
+  t.judge_Fd.alloc((size_t)std::max(J, 1) * OUT);
+  t.judge_acc.alloc(4);
+  t.judge_fail.alloc((size_t)t.ncells + 1);
+  double jscale = 0.0;
+  if (J > 0) {
+    PhaseTimer pt(t, kTJudge, st);
+    k_fill_zero<<<nblk((long long)J * OUT, 256), 256, 0, st>>>((long long)J * OUT, t.judge_Fd.get());
+    const dim3 g(J, (n + kJudgeSlice - 1) / kJudgeSlice);
+    k_judge_direct<<<g, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
+                                      t.judge_Fd.get());
+    std::vector<double> h((size_t)J * OUT);
+    FMM_HIP(hipMemcpyAsync(h.data(), t.judge_Fd.get(), sizeof(double) * J * OUT, hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
+    std::vector<double> m2(J);
+    for (int i = 0; i < J; ++i) {
+      double a = 0.0;
+      for (int k = FO; k < OUT; ++k) a += h[(size_t)i * OUT + k] * h[(size_t)i * OUT + k];
+      m2[i] = a;
+    }
+    // rms |F| of the sample without its largest 0.1% (at least its largest
+    // target: a near-coincident pair in the sample would dominate it)
+    std::sort(m2.begin(), m2.end());
+    const int keep = J - std::max(1, J / 1000);
+    double sum = 0.0;
+    for (int i = 0; i < keep; ++i) sum += m2[i];
+    jscale = keep > 0 ? sqrt(sum / keep) : 0.0;
+  }
   int peff;
+  double tol_eff = t.tol;
   {
     PhaseTimer pt(t, kTOrder, st);
     leaf_features(t, SS, OUT, FO, st);
+    // predictions are relative to F_near; the target is tol relative to
+    // the field of the judge's sample
+    if (jscale > 0.0 && t.near_scale > 0.0) tol_eff = t.tol * jscale / t.near_scale;
     t.leaf_q.alloc(std::max(nl, 1));
-    mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), nullptr, t.leaf_q.get(), (float)log10(t.tol),
-                  (float)t.margin, std::max(PMIN, kMlpOrder0), t.order, st);
+    t.leaf_y.alloc((size_t)std::max(nl, 1) * kMlpO);
+    mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), t.leaf_y.get(), nullptr, 0.f, 0.f, 0, 0, st);
+    t.lstats[7] = allocate_orders(t, t.leaf_y.get(), tol_eff, t.margin, qmin, t.order, st);
     peff = orders_from_leaves(t, st);
   }
   peff = std::max(std::min(peff, t.order), PMIN);
@@ -419,21 +471,8 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
     OrderTable<PMIN>::l2p(p, t, fmode, kPhaseAll, F, st);
   };
   far(peff);
-  // judge
-  const int J = std::min(kJudgeTargets, n);
   int fails = 0, reruns = 0;
   double jerr = 0.0;
-  if (t.judge_n != J) {
-    std::vector<int> h(J);
-    for (int i = 0; i < J; ++i) h[i] = (int)(((long long)(2 * i + 1) * n) / (2LL * J));
-    t.judge_slot.alloc(std::max(J, 1));
-    FMM_HIP(hipMemcpyAsync(t.judge_slot.get(), h.data(), sizeof(int) * J, hipMemcpyHostToDevice, st));
-    FMM_HIP(hipStreamSynchronize(st));
-    t.judge_n = J;
-  }
-  t.judge_Fd.alloc((size_t)std::max(J, 1) * OUT);
-  t.judge_acc.alloc(4);
-  t.judge_fail.alloc((size_t)t.ncells + 1);
   auto check = [&]() {
     k_fill_int<<<nblk(t.ncells + 1, 256), 256, 0, st>>>(t.ncells + 1, t.judge_fail.get());
     k_judge_check<<<1, 256, 0, st>>>(J, t.judge_slot.get(), t.perm.get(), t.leaf_of.get(), F, t.judge_Fd.get(),
@@ -449,18 +488,22 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   bool rerun = false;
   if (J > 0) {
     PhaseTimer pt(t, kTJudge, st);
-    k_judge_direct<<<J, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
-                                      t.judge_Fd.get());
     fails = check();
     rerun = jerr > t.tol;  // the sample's norm-wise error decides
   }
   if (rerun) {
     PhaseTimer pt(t, kTJudge, st);
-    // a few failures: their leaves rerun at the full order; more than one
-    // in kJudgeDistrust of the sample: the network is not trusted on this
-    // input and every leaf reruns at the full order
-    if ((long long)fails * kJudgeDistrust > J) raise_all_leaves(t, t.order, st);
-    else raise_failing_leaves(t, t.order, st);
+    // a few failures: their leaves rerun at the full order and the other
+    // leaves are re-allocated for half the tolerance (the sample says the
+    // predictions ran optimistic); more than one in kJudgeDistrust of the
+    // sample: the network is not trusted on this input and every leaf
+    // reruns at the full order
+    if ((long long)fails * kJudgeDistrust > J) {
+      raise_all_leaves(t, t.order, st);
+    } else {
+      allocate_orders(t, t.leaf_y.get(), 0.5 * tol_eff, t.margin, qmin, t.order, st);
+      raise_failing_leaves(t, t.order, st);
+    }
     peff = std::max(std::min(orders_from_leaves(t, st), t.order), PMIN);
     far(peff);
     reruns = 1;
@@ -472,6 +515,7 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   t.lstats[3] = jerr;
   t.lstats[4] = J;
   t.lstats[5] = m2l_mac_fraction(t, t.order, st, &t.lstats[6]);
+  t.lstats[8] = tol_eff / t.tol;
 }
 
 void op_batch(int op, int order, int n, const double* xyz, const double* in, int in_stride, double* out,

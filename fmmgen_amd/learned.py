@@ -1,13 +1,13 @@
 """Learned per-leaf expansion order: training data, training, model file.
 
 The order network (three layers, 64 hidden ReLU units, FP32 weights, FP16
-inference; kernels in csrc/mlp.hip) maps eight features of a leaf cell
+inference; kernels in csrc/mlp.hip) maps twelve features of a leaf cell
 (csrc/learned.h) to the predicted far-field error of the leaf at every
-order 2..12, as log10(rms |dF| over the leaf's particles / rms |F| over all
-targets): when every leaf meets the tolerance in this sense, the
-norm-wise relative error over all targets does too.  The
-learned evaluation (mode "learned") picks per leaf the lowest order whose
-prediction plus a safety margin meets the tolerance, runs every M2L tile
+order 2..12, as log10(rms |dF| over the leaf's particles / F_near), F_near
+the rms near-field magnitude over all targets.  The
+learned evaluation (mode "learned") picks the leaf orders with the fewest
+M2L multiply-adds whose predicted norm-wise error (predictions raised by a
+calibrated safety margin) meets the tolerance, runs every M2L tile
 at the largest order of its target cells, and checks a sample of targets
 against an FP64 direct sum, rerunning leaves that fail at the full order.
 
@@ -40,32 +40,54 @@ FLOOR = -16.0  # log10 of the smallest relative error kept (FP64 round-off)
 
 # ------------------------------------------------------------------ data
 
-def error_samples(pos, S, ncrit=64, theta=0.5, pmax=None):
-    """(X [nleaves, 8], Y [nleaves, 11]) float32 device tensors of one particle set.
+def error_samples(pos, S, ncrit=64, theta=0.5, pmax=None, max_leaves=8000, seed=0):
+    """(X [m, 12], Y [m, 11]) float32 device tensors of one particle set.
 
     Y[:, o] = log10(rms over the leaf's particles of |F_q - F_direct| /
-    rms |F_direct|) for q = ORDERS[o] (field components), from the plain
-    FP64 FMM at order q and the FP64 direct sum of every target.
+    F_near) for q = ORDERS[o] (field components), from the plain FP64 FMM
+    at order q and the FP64 direct sum; F_near is the rms near-field (P2P)
+    magnitude without the largest 0.1% (Tree.near_scale), the scale the
+    features use too, known before the far field runs.  Sets with more
+    than ``max_leaves`` leaves contribute a
+    random sample of that many leaves (the direct sum runs at their
+    particles only).
     """
+    from .tree import direct_sum
     pmax = pmax or ORDERS[-1]
     n = pos.shape[0]
     dpos = torch.from_numpy(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
     dS = torch.from_numpy(np.ascontiguousarray(S, dtype=np.float64)).cuda()
     tree = build_tree(dpos, dS, n, ncrit, pmax, theta)
-    Fd = tree.compute_field_exact(torch.empty(n, 4, dtype=torch.float64, device="cuda"))
-    fscale = torch.sqrt((Fd[:, 1:] ** 2).sum(1).mean()).item()
-    F = torch.empty_like(Fd)
+    leaves = tree.leaves()
+    nl = len(leaves)
+    F = torch.empty(n, 4, dtype=torch.float64, device="cuda")
+    if nl <= max_leaves:
+        rows = np.arange(nl)
+        Fd = tree.compute_field_exact(torch.empty(n, 4, dtype=torch.float64, device="cuda"))
+        tg = None
+    else:
+        rows = np.sort(np.random.default_rng(seed).choice(nl, max_leaves, replace=False))
+        _, leaf_of = tree.export_particles()
+        pick = np.zeros(int(leaves.max()) + 1, dtype=bool)
+        pick[leaves[rows]] = True
+        tg = torch.from_numpy(np.nonzero(pick[leaf_of])[0].astype(np.int64)).cuda()
+        Fsub = direct_sum(dpos, dS, tg)
+    tree.compute_field_fmm(F, mode="plain", phases=_lib.PHASE_P2P)
+    X = tree.leaf_features()[torch.from_numpy(rows).cuda()]
+    fscale = tree.near_scale()
     Y = []
     for q in ORDERS:
         tree.set_order(q)
         tree.compute_field_fmm(F, mode="plain")
-        e = tree.leaf_error(F, Fd) / fscale
+        if tg is not None:  # reference: the FMM itself, direct sums at the sampled leaves
+            Fd = F.clone()
+            Fd[tg] = Fsub
+        e = tree.leaf_error(F, Fd)[torch.from_numpy(rows).cuda()] / fscale
         Y.append(torch.log10(torch.clamp(e, min=10.0 ** FLOOR)))
-    X = tree.leaf_features()
     return X, torch.stack(Y, 1).float()
 
 
-def problem_set(total, seed=0, nmin=1 << 16, nmax=1 << 18):
+def problem_set(total, seed=0, nmin=1 << 16, nmax=1 << 22):
     """Particle sets of mixed kinds and sizes until ``total`` leaves (samples) are reached."""
     rng = np.random.default_rng(seed)
     kinds = ["blobs", "blobs", "uniform", "plummer", "mixed", "blobs_tight", "blobs_wide"]
@@ -90,7 +112,7 @@ def make_dataset(total, seed=0, log=print, **kw):
     Xs, Ys, Ps, got = [], [], [], 0
     t0 = time.time()
     for k, (kind, pos, S) in enumerate(problem_set(total, seed, **kw)):
-        X, Y = error_samples(pos, S)
+        X, Y = error_samples(pos, S, seed=seed + k)
         Xs.append(X)
         Ys.append(Y)
         Ps.append(torch.full((X.shape[0],), k, dtype=torch.int32, device=X.device))
@@ -194,29 +216,59 @@ def train(X, Y, epochs=40, batch=2048, lr=3e-3, seed=0, log=print):
     return fold_normalisation(params, mu, sigma)
 
 
-def problem_errors(Y_true, Y_pred, counts, pid, tol, margin):
-    """Norm-wise relative field error of every held-out particle set when each
-    leaf runs at its chosen order: sqrt(sum_l n_l e_l^2 / sum_l n_l), with
-    e_l the leaf's rms error relative to the set's rms field."""
-    lt = np.log10(tol)
-    ok = Y_pred + margin <= lt
-    o = np.where(ok.any(1), ok.argmax(1), O - 1)
-    e = 10.0 ** Y_true[np.arange(len(o)), o]
-    num = np.bincount(pid, weights=counts * e * e)
-    den = np.bincount(pid, weights=counts)
-    keep = den > 0
-    return np.sqrt(num[keep] / den[keep])
+def allocate(Yp, counts, w, tol, margin, qmin=ORDER0, qmax=ORDERS[-1]):
+    """Column (order - ORDER0) per leaf chosen as the device does
+    (csrc/learned.hip, allocate_orders): the cheapest orders, by
+    C(q + 6, 6) w_l + lam n_l e_l(q)^2, whose predicted norm-wise error
+    meets tol, lam found by bisection on log10 lam in [-10, 40]."""
+    from math import comb
+    cols = np.array([o for o, q in enumerate(ORDERS) if qmin <= q <= qmax])
+    macs = np.array([comb(ORDERS[o] + 6, 6) for o in cols], dtype=np.float64)
+    e2 = 10.0 ** (2.0 * (Yp[:, cols] + margin))
+    target = tol * tol * counts.sum()
+
+    def pick(loglam):
+        c = macs[None, :] * w[:, None] + 10.0 ** loglam * counts[:, None] * e2
+        o = c.argmin(1)
+        return o, (counts * e2[np.arange(len(o)), o]).sum()
+
+    lo, hi = -10.0, 40.0
+    o, e = pick(hi)
+    if e > target:
+        return cols[o]
+    for _ in range(34):
+        mid = 0.5 * (lo + hi)
+        if pick(mid)[1] <= target:
+            hi = mid
+        else:
+            lo = mid
+    return cols[pick(hi)[0]]
 
 
-def calibrate_margin(Y_true, Y_pred, X, pid, tol, target=0.5):
+def problem_errors(Y_true, Y_pred, X, pid, tol, margin):
+    """True norm-wise relative field error of every particle set when its
+    leaves run at the allocated orders: sqrt(sum_l n_l e_l^2 / sum_l n_l),
+    e_l the leaf's true rms error relative to the set's rms field."""
+    counts = np.exp2(X[:, 6].astype(np.float64))
+    w = np.maximum(10.0 ** X[:, 5].astype(np.float64) - 1.0, 1.0)
+    out = []
+    for k in np.unique(pid):
+        s = pid == k
+        o = allocate(Y_pred[s], counts[s], w[s], tol, margin)
+        e = 10.0 ** Y_true[s][np.arange(s.sum()), o]
+        out.append(np.sqrt((counts[s] * e * e).sum() / counts[s].sum()))
+    return np.array(out)
+
+
+def calibrate_margin(Y_true, Y_pred, X, pid, tol, target=0.8):
     """Smallest margin (log10 units, step 0.05) for which every held-out
-    particle set meets target * tol norm-wise."""
+    particle set meets target * tol norm-wise at the allocated orders."""
     yt = Y_true.cpu().numpy().astype(np.float64)
     yp = Y_pred.cpu().numpy().astype(np.float64)
-    counts = np.exp2(X[:, 6].cpu().numpy().astype(np.float64))
+    x = X.cpu().numpy()
     p = pid.cpu().numpy()
-    for margin in np.arange(-1.0, 3.0, 0.05):
-        if problem_errors(yt, yp, counts, p, tol, margin).max() <= target * tol:
+    for margin in np.arange(-0.5, 3.0, 0.05):
+        if problem_errors(yt, yp, x, p, tol, margin).max() <= target * tol:
             return float(margin)
     return 3.0
 
@@ -254,9 +306,9 @@ def main():
     rmse = torch.sqrt(((Yp - Yh) ** 2).mean()).item()
     margin = calibrate_margin(Yh, Yp, Xh, ph, a.tol)
     errs = problem_errors(Yh.cpu().numpy().astype(np.float64), Yp.cpu().numpy().astype(np.float64),
-                          np.exp2(Xh[:, 6].cpu().numpy().astype(np.float64)), ph.cpu().numpy(), a.tol, margin)
+                          Xh.cpu().numpy(), ph.cpu().numpy(), a.tol, margin)
     print(f"held-out rmse {rmse:.3f} (log10 units), margin {margin:.2f}, held-out set errors "
-          f"max {errs.max():.2e} median {np.median(errs):.2e}", flush=True)
+          f"max {errs.max():.2e} median {np.median(errs):.2e}: {np.array2string(errs, precision=2)}", flush=True)
     save(a.out, params, margin, a.tol,
          f"samples={X.shape[0]} holdout={Xh.shape[0]} rmse={rmse:.4f} epochs={a.epochs} seed={a.seed}")
     print(f"saved {a.out} ({time.time() - t0:.1f}s)", flush=True)

@@ -81,11 +81,17 @@ class Tree:
         return out
 
     def leaf_features(self):
-        """[nleaves, 8] float32 device tensor of the leaf features (needs a
+        """[nleaves, 12] float32 device tensor of the leaf features (needs a
         previous evaluation that ran P2P)."""
         f = torch.empty(self.nleaves(), _lib.MLP_FEATURES, dtype=torch.float32, device="cuda")
         _lib.check(_lib.load().fmm_tree_leaf_features(self._h, _ptr(f), _stream()))
         return f
+
+    def near_scale(self):
+        """F_near of the last leaf_features call (the network's output scale)."""
+        v = ctypes.c_double()
+        _lib.check(_lib.load().fmm_tree_near_scale(self._h, ctypes.byref(v)))
+        return v.value
 
     def leaf_error(self, F, Fref):
         """[nleaves] float64 device tensor: the largest |F - Fref| (field
@@ -103,10 +109,10 @@ class Tree:
         _lib.check(_lib.load().fmm_tree_set_model(self._h, _ptr(params.contiguous()), float(tol), float(margin)))
 
     def learned_stats(self):
-        a = (ctypes.c_double * 8)()
+        a = (ctypes.c_double * 10)()
         _lib.check(_lib.load().fmm_tree_learned_stats(self._h, a))
         keys = ("order_max", "judge_failures", "reruns", "judge_rel_err", "judge_targets", "m2l_mac_fraction",
-                "m2l_macs")
+                "m2l_macs", "predicted_rel_err", "tol_scale")
         return dict(zip(keys, list(a)))
 
     def leaf_orders(self):

@@ -143,14 +143,15 @@ int fmm_export_expansions(fmm_tree_t tree, double* h_M, double* h_L);
  * Network: FMM_MLP_FEATURES inputs, two hidden ReLU layers of
  * FMM_MLP_HIDDEN units, FMM_MLP_OUTPUTS outputs; FP32 parameters, flat
  * [W1 (H x F), b1 (H), W2 (H x H), b2 (H), W3 (O x H), b3 (O)], row-major.
- * Output o predicts log10(rms |dF| over a leaf's particles / rms |F| over
- * all targets) of the far field at order FMM_MLP_ORDER0 + o.  All pointers are device
+ * Output o predicts log10(rms |dF| over a leaf's particles / F_near) of
+ * the far field at order FMM_MLP_ORDER0 + o, F_near the rms near-field
+ * (P2P) magnitude (fmm_tree_near_scale).  All pointers are device
  * pointers except where named h_. */
-#define FMM_MLP_FEATURES 8
+#define FMM_MLP_FEATURES 12
 #define FMM_MLP_HIDDEN 64
 #define FMM_MLP_OUTPUTS 11
 #define FMM_MLP_ORDER0 2
-#define FMM_MLP_PARAMS 5451
+#define FMM_MLP_PARAMS 5707
 
 /* FP32 forward: d_Y [B*O]. */
 int fmm_mlp_forward(const float* d_X, int B, const float* d_params, float* d_Y, void* stream);
@@ -176,6 +177,10 @@ int fmm_tree_set_order(fmm_tree_t tree, int order);
 /* Leaf features d_feat [nleaves*FMM_MLP_FEATURES], leaves in canonical
  * order; they use the near field of the last evaluation that ran P2P. */
 int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream);
+/* F_near of the last feature pass (rms near-field magnitude over the
+ * targets outside the largest 0.1% of magnitudes), the scale of the
+ * network's outputs: h_scale [1]. */
+int fmm_tree_near_scale(fmm_tree_t tree, double* h_scale);
 /* Root-mean-square |F - Fref| (field components) over each leaf's
  * particles: d_err [nleaves]; d_F, d_Fref [n*OUTPUTSIZE] in caller order. */
 int fmm_tree_leaf_error(fmm_tree_t tree, const double* d_F, const double* d_Fref, double* d_err, void* stream);
@@ -184,13 +189,14 @@ int fmm_tree_leaves(fmm_tree_t tree, long long* h_leaves);
 /* Network (copied, converted to FP16), tolerance and safety margin (in
  * log10 units) of the learned modes. */
 int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, double margin);
-/* Statistics of the last learned evaluation, h_stats [8]: largest order
+/* Statistics of the last learned evaluation, h_stats [10]: largest order
  * used, judge targets above tol * rms |F| (a rerun happens when the
  * sample's rel. field error exceeds tol), reruns (0/1), judge rel. field
- * error (after the rerun, if any),
- * judge targets, M2L multiply-adds relative to the tree's order, M2L
- * useful multiply-adds (sum over pairs of C(q + 6, 6) at the pair's
- * order), 0. */
+ * error (after the rerun, if any), judge targets, M2L multiply-adds
+ * relative to the tree's order, M2L useful multiply-adds (sum over pairs of
+ * C(q + 6, 6) at the pair's order), predicted error of the chosen orders
+ * relative to F_near, the factor applied to tol for the allocation (the
+ * judge sample's field scale over F_near), 0. */
 int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats);
 /* Per-leaf orders of the last learned evaluation: h_order [nleaves]. */
 int fmm_tree_leaf_orders(fmm_tree_t tree, int* h_order);

@@ -13,7 +13,7 @@ Parameter layout (flat): W1[H][F], b1[H], W2[H][H], b2[H], W3[O][H], b3[O].
 """
 import numpy as np
 
-F, H, O = 8, 64, 11
+F, H, O = 12, 64, 11
 SIZES = [("W1", (H, F)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (O, H)), ("b3", (O,))]
 NPARAMS = sum(int(np.prod(s)) for _, s in SIZES)
 

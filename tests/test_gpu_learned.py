@@ -118,36 +118,42 @@ def _cpu_features(tree, q, Fnear):
     perm, leaf = tree.export_particles()
     nc = geo.shape[0]
     qabs = np.zeros(nc)
+    q2 = np.zeros(nc)
     np.add.at(qabs, leaf, np.abs(q[:, 0]))
+    np.add.at(q2, leaf, q[:, 0] ** 2)
     # sums over subtrees: children have larger ids than parents
-    sub = qabs.copy()
     for c in range(nc - 1, 0, -1):
-        sub[meta[c, 0]] += sub[c]
-    acc = np.zeros((nc, 7))
+        qabs[meta[c, 0]] += qabs[c]
+        q2[meta[c, 0]] += q2[c]
+    acc = np.zeros((nc, 11))
     src, tgt = m2l[:, 0], m2l[:, 1]
     d = geo[tgt, :3] - geo[src, :3]
     R2 = (d ** 2).sum(1)
     rho = (geo[tgt, 4] + geo[src, 4]) / np.sqrt(R2)
-    w = sub[src] / R2
+    w = qabs[src] / R2
+    v = np.sqrt(q2[src]) / R2
     for k, e in enumerate((0, 3, 6, 9, 12)):
         np.add.at(acc[:, k], tgt, w * rho ** e)
-    np.maximum.at(acc[:, 5], tgt, rho)
-    np.add.at(acc[:, 6], tgt, 1.0)
+    for k, e in enumerate((3, 6, 9, 12)):
+        np.add.at(acc[:, 5 + k], tgt, v * rho ** e)
+    np.maximum.at(acc[:, 9], tgt, rho)
+    np.add.at(acc[:, 10], tgt, 1.0)
     for c in range(1, nc):                 # root first
         p = meta[c, 0]
-        acc[c, :5] += acc[p, :5]
-        acc[c, 5] = max(acc[c, 5], acc[p, 5])
-        acc[c, 6] += acc[p, 6]
-    fn = np.sqrt((Fnear[:, 1:] ** 2).sum() / len(Fnear))
+        acc[c, :9] += acc[p, :9]
+        acc[c, 9] = max(acc[c, 9], acc[p, 9])
+        acc[c, 10] += acc[p, 10]
+    fn = tree.near_scale()
     leaves = tree.leaves()
     a = acc[leaves]
     lg = lambda v: np.clip(np.log10(np.maximum(v, 1e-300) / fn), -40, 20)
-    f = np.zeros((len(leaves), 8))
+    f = np.zeros((len(leaves), 12))
     f[:, 0:4] = lg(a[:, 1:5])
-    f[:, 4] = a[:, 5]
-    f[:, 5] = np.log10(1 + a[:, 6])
+    f[:, 4] = a[:, 9]
+    f[:, 5] = np.log10(1 + a[:, 10])
     f[:, 6] = np.log2(np.maximum(meta[leaves, 3], 1))
     f[:, 7] = lg(a[:, 0])
+    f[:, 8:12] = lg(a[:, 5:9])
     return f
 
 
@@ -162,6 +168,11 @@ def test_leaf_features_and_errors_match_numpy(gpu):
     fg = t.leaf_features().cpu().numpy()
     fc = _cpu_features(t, q, Fn.cpu().numpy())
     assert np.abs(fg - fc).max() <= 1e-4 * max(1.0, np.abs(fc).max())
+    # the scale: rms |F_near| without the largest 0.1% (histogram bins of 2^(1/4))
+    m = np.sort(np.sqrt((Fn.cpu().numpy()[:, 1:] ** 2).sum(1)))
+    lo = np.sqrt((m[: n - int(0.001 * n)] ** 2).mean())
+    hi = np.sqrt((m ** 2).mean())
+    assert lo / 1.2 <= t.near_scale() <= hi
     # per-leaf rms error
     F = torch.zeros_like(Fn)
     t.compute_field_fmm(F, mode="plain")
