@@ -49,6 +49,9 @@ void raise_failing_leaves(Tree& t, int qfull, hipStream_t st);
 // start[q] .. start[q + 1] is order q's range (start has
 // M2LSchedule::kMaxOrder + 1 entries, host).
 void chunk_classes(Tree& t, hipStream_t st, int* start);
+// Sum over M2L pairs of C(q + 6, 6) at the order of the pair's target
+// cell (what the truncated M2L would cost without the tile maximum).
+double m2l_cell_macs(Tree& t, hipStream_t st);
 // Every leaf moves to order qfull.
 void raise_all_leaves(Tree& t, int qfull, hipStream_t st);
 // Sum over M2L pairs of C(q + 6, 6) at their tile's order (the useful

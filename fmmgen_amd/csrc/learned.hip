@@ -311,7 +311,33 @@ __global__ void k_alloc(int nl, const float* __restrict__ feat, const float* __r
   if (threadIdx.x == 0) atomicAdd(err_sum, red[0]);
 }
 
+// sum over target cells of (M2L pairs) x C(q_cell + 6, 6)
+__global__ void k_cell_macs(int ncells, const int* __restrict__ off, const int* __restrict__ cq,
+                            double* __restrict__ out) {
+  __shared__ double red[256];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  red[threadIdx.x] = c < ncells ? (double)(off[c + 1] - off[c]) * macs(cq[c]) : 0.0;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
 }  // namespace
+
+double m2l_cell_macs(Tree& t, hipStream_t st) {
+  const int nc = t.ncells;
+  DevBuf<double> acc;
+  acc.alloc(1);
+  k_zero_d<<<1, 64, 0, st>>>(1, acc.get());
+  k_cell_macs<<<nb(nc, 256), 256, 0, st>>>(nc, t.m2l_off.get(), t.cell_q.get(), acc.get());
+  double h = 0.0;
+  FMM_HIP(hipMemcpyAsync(&h, acc.get(), sizeof(double), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  return h;
+}
 
 double allocate_orders(Tree& t, const float* Y, double tol, double margin, int qmin, int qmax, hipStream_t st) {
   const int nl = t.nleaves;

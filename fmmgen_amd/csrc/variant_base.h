@@ -516,6 +516,7 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   t.lstats[4] = J;
   t.lstats[5] = m2l_mac_fraction(t, t.order, st, &t.lstats[6]);
   t.lstats[8] = tol_eff / t.tol;
+  t.lstats[9] = m2l_cell_macs(t, st);
 }
 
 void op_batch(int op, int order, int n, const double* xyz, const double* in, int in_stride, double* out,

@@ -260,7 +260,7 @@ def problem_errors(Y_true, Y_pred, X, pid, tol, margin):
     return np.array(out)
 
 
-def calibrate_margin(Y_true, Y_pred, X, pid, tol, target=0.8):
+def calibrate_margin(Y_true, Y_pred, X, pid, tol, target=1.0):
     """Smallest margin (log10 units, step 0.05) for which every held-out
     particle set meets target * tol norm-wise at the allocated orders."""
     yt = Y_true.cpu().numpy().astype(np.float64)

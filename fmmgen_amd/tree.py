@@ -112,7 +112,7 @@ class Tree:
         a = (ctypes.c_double * 10)()
         _lib.check(_lib.load().fmm_tree_learned_stats(self._h, a))
         keys = ("order_max", "judge_failures", "reruns", "judge_rel_err", "judge_targets", "m2l_mac_fraction",
-                "m2l_macs", "predicted_rel_err", "tol_scale")
+                "m2l_macs", "predicted_rel_err", "tol_scale", "m2l_cell_macs")
         return dict(zip(keys, list(a)))
 
     def leaf_orders(self):
