@@ -34,7 +34,7 @@ EXPORTS = (
     "fmm_timing_enable", "fmm_timing_read",
     "fmm_mlp_forward", "fmm_mlp_infer_f16", "fmm_mlp_work_floats", "fmm_mlp_grad", "fmm_mlp_adam",
     "fmm_tree_set_order", "fmm_tree_leaf_features", "fmm_tree_leaf_error", "fmm_tree_leaves",
-    "fmm_tree_set_model", "fmm_tree_learned_stats", "fmm_tree_leaf_orders", "fmm_tree_near_scale",
+    "fmm_tree_set_model", "fmm_tree_learned_stats", "fmm_tree_leaf_orders", "fmm_tree_field_scale",
 )
 PHASE_NAMES = ("prep", "P2M", "M2M", "M2L", "P2P", "L2L", "L2P", "order", "judge", "unused")
 
@@ -100,7 +100,7 @@ def load(path=None):
     L.fmm_tree_set_model.argtypes = [vp, vp, cd, cd]
     L.fmm_tree_learned_stats.argtypes = [vp, d_p]
     L.fmm_tree_leaf_orders.argtypes = [vp, i_p]
-    L.fmm_tree_near_scale.argtypes = [vp, d_p]
+    L.fmm_tree_field_scale.argtypes = [vp, d_p, vp]
     for nm in EXPORTS:
         if nm not in ("fmm_last_error",):
             getattr(L, nm).restype = ci

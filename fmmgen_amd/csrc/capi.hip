@@ -148,6 +148,9 @@ int fmm_tree_destroy(fmm_tree_t tree) {
         (void)hipEventDestroy(e.second.second);
       }
       for (auto e : tree->t.event_pool) (void)hipEventDestroy(e);
+      if (tree->t.ev_gather) (void)hipEventDestroy(tree->t.ev_gather);
+      if (tree->t.ev_near) (void)hipEventDestroy(tree->t.ev_near);
+      if (tree->t.side) (void)hipStreamDestroy(tree->t.side);
     }
     delete tree;
   });
@@ -392,9 +395,10 @@ int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream) {
   return guarded([&] {
     FMM_CHECK(tree && d_feat, kInvalidArgument, "NULL argument");
     Tree& t = tree->t;
-    FMM_CHECK(t.Fp2p.get() && t.pS.get(), kInvalidArgument, "no evaluation with P2P has run on this tree");
+    FMM_CHECK(t.pS.get() != nullptr, kInvalidArgument, "no evaluation has run on this tree");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    leaf_features(t, sourcesize(t.so), t.outsz, t.outsz == 4 ? 1 : 0, st);
+    tree->v->field_scale(t, st);
+    leaf_features(t, sourcesize(t.so), st);
     if (t.nleaves > 0)
       FMM_HIP(hipMemcpyAsync(d_feat, t.leaf_feat.get(), sizeof(float) * t.nleaves * kMlpF,
                              hipMemcpyDeviceToDevice, st));
@@ -402,10 +406,13 @@ int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream) {
   });
 }
 
-int fmm_tree_near_scale(fmm_tree_t tree, double* h_scale) {
+int fmm_tree_field_scale(fmm_tree_t tree, double* h_scale, void* stream) {
   return guarded([&] {
     FMM_CHECK(tree && h_scale, kInvalidArgument, "NULL argument");
-    *h_scale = tree->t.near_scale;
+    Tree& t = tree->t;
+    FMM_CHECK(t.pS.get() != nullptr, kInvalidArgument, "no evaluation has run on this tree");
+    tree->v->field_scale(t, static_cast<hipStream_t>(stream));
+    *h_scale = t.judge_scale;
   });
 }
 

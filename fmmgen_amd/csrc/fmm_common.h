@@ -183,7 +183,11 @@ struct Tree {
   DevBuf<float> leaf_y;            // [nleaves * kMlpO] network outputs
   DevBuf<double> cell_qabs, cell_q2;  // [ncells] sum of |S| and of |S|^2 below the cell
   DevBuf<double> cell_acc;            // [ncells * 11] list sums, accumulated down the tree
-  double near_scale = 0.0;         // F_near of the last feature pass
+  double judge_scale = 0.0;        // field scale of the judge sample (network output scale)
+  // side stream of the learned evaluation (P2P runs beside order selection
+  // and the far field) and its two events; created on first use
+  hipStream_t side = nullptr;
+  hipEvent_t ev_gather = nullptr, ev_near = nullptr;
   DevBuf<signed char> tile_order;  // [ntiles]
   bool use_tile_order = false;
   const signed char* tile_order_ptr() const { return use_tile_order ? tile_order.get() : nullptr; }

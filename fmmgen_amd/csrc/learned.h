@@ -4,38 +4,36 @@
 // the leaf: with rho = (rmax_A + rmax_B) / R, w = sum|S|_B / R^2 and
 // v = sqrt(sum |S|^2)_B / R^2 over every M2L pair (B -> A) of the leaf and
 // of its ancestors (v is the size of a multipole of random-sign sources),
-//   f0..f3   log10(sum w rho^k / F_near), k = 3, 6, 9, 12
+//   f0..f3   log10(sum w rho^k / F_s), k = 3, 6, 9, 12
 //   f4       largest rho
 //   f5       log10(1 + number of those pairs)
 //   f6       log2(particles in the leaf)
-//   f7       log10(sum w / F_near)
-//   f8..f11  log10(sum v rho^k / F_near), k = 3, 6, 9, 12
-// where F_near is the rms near-field (P2P) magnitude of the evaluation over
-// the targets outside the largest 0.1% of magnitudes (the field of nearly
-// coincident pairs would dominate a plain rms, and a sample of targets
-// seldom holds one).  The
-// network maps them to the predicted rms far-field error of the leaf per
-// order (relative to F_near); the leaf orders are
-// the cheapest set whose predicted norm-wise error over all targets meets
-// the tolerance; a cell's
-// order is the largest order of the leaves below it, and an M2L tile runs
-// at the largest order of its target cells.
+//   f7       log10(sum w / F_s)
+//   f8..f11  log10(sum v rho^k / F_s), k = 3, 6, 9, 12
+// where F_s is the field scale of the evaluation's judge sample: the rms
+// |F| of the FP64 direct sums at the judge's stratified targets without
+// the largest 0.1% of them (at least the largest; the field of nearly
+// coincident pairs would dominate a plain rms).  The network maps the
+// features to the predicted rms far-field error of the leaf per order,
+// relative to F_s; the leaf orders are the cheapest set whose predicted
+// norm-wise error over all targets meets the tolerance; a cell's order is
+// the largest order of the leaves below it, and an M2L tile runs at the
+// largest order of its target cells.
 #pragma once
 #include "fmm_common.h"
 #include "mlp.h"
 
 namespace fmm {
 
-// Features of every leaf (canonical leaf order) into t.leaf_feat; uses the
-// sorted near field t.Fp2p (outsz values per target, field components
-// from fo) and sorted source strengths t.pS (ss per particle).
-void leaf_features(Tree& t, int ss, int outsz, int fo, hipStream_t st);
+// Features of every leaf (canonical leaf order) into t.leaf_feat, from the
+// sorted source strengths t.pS (ss per particle) and t.judge_scale.
+void leaf_features(Tree& t, int ss, hipStream_t st);
 // Root-mean-square |F - Fref| (field components) over the particles of
 // each leaf; F and Fref in caller order.
 void leaf_error(Tree& t, const double* F, const double* Fref, int outsz, int fo, double* err, hipStream_t st);
 // Leaf orders into t.leaf_q from the network outputs Y [nleaves][O]: the
 // cheapest orders (M2L multiply-adds) whose predicted norm-wise field
-// error relative to F_near, sqrt(sum_l n_l e_l^2 / n) with
+// error relative to F_s, sqrt(sum_l n_l e_l^2 / n) with
 // e_l = 10^(Y_l + margin), meets tol (a Lagrange multiplier found by
 // bisection); returns that predicted error.
 double allocate_orders(Tree& t, const float* Y, double tol, double margin, int qmin, int qmax, hipStream_t st);

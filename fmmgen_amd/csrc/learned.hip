@@ -117,36 +117,6 @@ __global__ void k_acc_down(int beg, int end, const int* __restrict__ parent, dou
   a[10] += p[10];
 }
 
-// Histogram of log2 |F| (field components) of the sorted near field:
-// kHistBins bins of width 1/4 from 2^kHistLo, counts and sums of |F|^2.
-constexpr int kHistBins = 512;
-constexpr double kHistLo = -64.0;
-
-__global__ void k_field_hist(long long n, int outsz, int fo, const double* __restrict__ F,
-                             unsigned long long* __restrict__ cnt, double* __restrict__ s2) {
-  __shared__ unsigned int lc[kHistBins];
-  __shared__ float ls[kHistBins];
-  for (int b = threadIdx.x; b < kHistBins; b += blockDim.x) {
-    lc[b] = 0u;
-    ls[b] = 0.f;
-  }
-  __syncthreads();
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double m2 = 0.0;
-    for (int k = fo; k < outsz; ++k) m2 += F[i * outsz + k] * F[i * outsz + k];
-    int b = m2 > 0.0 ? (int)floor((0.5 * log2(m2) - kHistLo) * 4.0) : 0;
-    b = b < 0 ? 0 : (b >= kHistBins ? kHistBins - 1 : b);
-    atomicAdd(&lc[b], 1u);
-    atomicAdd(&ls[b], (float)(m2 * exp2(-2.0 * (kHistLo + 0.25 * b))));  // scaled into [1, 2^0.5)^2
-  }
-  __syncthreads();
-  for (int b = threadIdx.x; b < kHistBins; b += blockDim.x)
-    if (lc[b]) {
-      atomicAdd(&cnt[b], (unsigned long long)lc[b]);
-      atomicAdd(&s2[b], (double)ls[b] * exp2(2.0 * (kHistLo + 0.25 * b)));
-    }
-}
-
 __global__ void k_leaf_feat(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pcount,
                             const double* __restrict__ acc, double fn, float* __restrict__ feat) {
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,7 +366,7 @@ void chunk_classes(Tree& t, hipStream_t st, int* start) {
   }
 }
 
-void leaf_features(Tree& t, int ss, int outsz, int fo, hipStream_t st) {
+void leaf_features(Tree& t, int ss, hipStream_t st) {
   const int nc = t.ncells, nl = t.nleaves;
   t.cell_qabs.alloc(nc);
   t.cell_q2.alloc(nc);
@@ -418,34 +388,7 @@ void leaf_features(Tree& t, int ss, int outsz, int fo, hipStream_t st) {
     const int b = t.level_off[l], e = t.level_off[l + 1];
     k_acc_down<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.parent.get(), t.cell_acc.get());
   }
-  // F_near: rms |F| of the near field over the targets below the top
-  // 0.1% of magnitudes (near-coincident pairs would dominate a plain rms)
-  const long long n = (long long)t.n;
-  DevBuf<unsigned long long> hc;
-  DevBuf<double> hs;
-  hc.alloc(kHistBins);
-  hs.alloc(kHistBins);
-  k_zero_d<<<nb(kHistBins, 256), 256, 0, st>>>(kHistBins, reinterpret_cast<double*>(hc.get()));
-  k_zero_d<<<nb(kHistBins, 256), 256, 0, st>>>(kHistBins, hs.get());
-  k_field_hist<<<std::min(nb(n, 256), 1024), 256, 0, st>>>(n, outsz, fo, t.Fp2p.get(), hc.get(), hs.get());
-  std::vector<unsigned long long> c(kHistBins);
-  std::vector<double> s2(kHistBins);
-  FMM_HIP(hipMemcpyAsync(c.data(), hc.get(), sizeof(unsigned long long) * kHistBins, hipMemcpyDeviceToHost, st));
-  FMM_HIP(hipMemcpyAsync(s2.data(), hs.get(), sizeof(double) * kHistBins, hipMemcpyDeviceToHost, st));
-  FMM_HIP(hipStreamSynchronize(st));
-  const unsigned long long drop = (unsigned long long)(0.001 * (double)n);
-  unsigned long long above = 0, kept = 0;
-  double sum = 0.0;
-  for (int b = kHistBins - 1; b >= 0; --b) {
-    if (above + c[b] <= drop) {
-      above += c[b];
-      continue;
-    }
-    kept += c[b];
-    sum += s2[b];
-  }
-  t.near_scale = kept > 0 ? sqrt(sum / (double)kept) : 1.0;
-  k_leaf_feat<<<nb(nl, 128), 128, 0, st>>>(nl, t.leaves.get(), t.pcount.get(), t.cell_acc.get(), t.near_scale,
+  k_leaf_feat<<<nb(nl, 128), 128, 0, st>>>(nl, t.leaves.get(), t.pcount.get(), t.cell_acc.get(), t.judge_scale,
                                           t.leaf_feat.get());
 }
 

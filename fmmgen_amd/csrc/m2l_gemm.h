@@ -128,8 +128,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 }
 
 // Steps of wave wv of a W-wave block (row blocks I = wv mod W), ordered by
-// the lowest total degree deg(row block) + deg(slice) so that a group
-// truncated at order q runs a prefix: cnt[q] steps.
+// the lowest total degree deg(row block) + deg(slice) (cnt[q]: the steps of
+// lowest degree <= q).
 template <int P, int SO, int W, int WV>
 struct WavePlan {
   using Plan = M2LPlan<P, SO>;
@@ -171,8 +171,7 @@ __device__ __forceinline__ void dispatch_wave(int w, F&& f) {
 }
 
 // tile_order (optional, per tile): the order q <= P the pairs of the tile's
-// groups are truncated at: D entries of degree > q are zeroed and the
-// steps whose lowest degree exceeds q are not issued.  chunk_list
+// groups are truncated at (D entries of degree > q are zeroed).  chunk_list
 // (optional): the chunks to run (else 0 .. nchunks-1).  mstride / lstride:
 // row strides of M and the D table / of L (the sizes of a higher order
 // when the expansions are stored for one: the layout is a prefix).
@@ -225,20 +224,26 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 #pragma unroll
     for (int i = 0; i < (NIW > 0 ? NIW : 1); ++i) acc[i] = V4{0, 0, 0, 0};
     T rD[DJ], rB[CPW][NJ];
+    // loads are issued unconditionally (empty columns read cell 0, slots
+    // past MS read the row's last slot) and masked afterwards: no branch
+    // around a load
     auto prefetch = [&](int g) {
       const int key = group_key[g];
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        rD[i] = j < MS ? Dtab[(size_t)key * mstride + j] : T(0);
+        const T v = Dtab[(size_t)key * mstride + (j < MS ? j : MS - 1)];
+        rD[i] = j < MS ? v : T(0);
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c) {
         const int src = group_src[(size_t)g * kTile + wv + W * c];
+        const T* row = M + (size_t)(src >= 0 ? src : 0) * mstride;
 #pragma unroll
         for (int i = 0; i < NJ; ++i) {
           const int j = 64 * i + lane;
-          rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * mstride + j] : T(0);
+          const T v = row[j < MS ? j : MS - 1];
+          rB[c][i] = (src >= 0 && j < MS) ? v : T(0);
         }
       }
     };
@@ -264,12 +269,13 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
         }
       __syncthreads();  // staged operands visible to every wave
       if (g + 1 < g1) prefetch(g + 1);  // in flight during the MFMAs below
-      const int cnt = WP::tab.cnt[qc];
       // batches of kBatch steps: all operand reads of a batch, then its
-      // MFMAs; a batch past the group's order is skipped
+      // MFMAs (a tile of a lower order than P has its D entries of higher
+      // degree zeroed; the learned evaluation launches each tile order with
+      // the kernel of that order, so no step is wasted there)
       constexpr int kBatch = 16;
       static_for<(NSW + kBatch - 1) / kBatch>([&](auto bb) {
-        if (bb * kBatch < cnt) {
+        {
           T av[kBatch], bv[kBatch];
           static_for<kBatch>([&](auto u) {
             constexpr int i = bb * kBatch + u;

@@ -364,12 +364,12 @@ void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
   OrderTable<PMIN>::l2p(t.order, t, mode, mask, F, st);
 }
 
-// Learned-order evaluation: FP32 P2P, leaf features, FP16 network
-// inference -> leaf orders (the cheapest whose predicted norm-wise error
-// meets the tolerance) -> cell and tile orders, mixed-precision far
-// field at the largest order with every M2L tile truncated at its own
-// order, then the sampled judge: an FP64 direct sum at kJudgeTargets
-// targets.  When the sample's relative field error exceeds the tolerance,
+// Learned-order evaluation: the judge sample's FP64 direct sums and field
+// scale, leaf features, FP16 network inference -> leaf orders (the
+// cheapest whose predicted norm-wise error meets the tolerance) -> cell
+// and tile orders, mixed-precision far field at the largest order with
+// every M2L tile truncated at its own order (FP32 P2P on a side stream
+// meanwhile), then the judge's check at its kJudgeTargets targets.  When the sample's relative field error exceeds the tolerance,
 // the leaves holding a target whose error exceeds tol * rms |F| rerun at
 // the tree's order and the other leaves are re-allocated for half the
 // tolerance (all leaves run at the tree's order when more than one target
@@ -391,21 +391,13 @@ void m2l_by_class(Tree& t, int mode, int P, hipStream_t st) {
 constexpr int kJudgeTargets = 1024;
 constexpr int kJudgeDistrust = 64;
 
-void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
-  FMM_CHECK(t.has_model, kInvalidArgument, "no order model set on the tree");
+// Judge sample and field scale: FP64 direct sums at kJudgeTargets
+// stratified sorted slots (t.judge_Fd) and t.judge_scale, their rms |F|
+// without the largest 0.1% (at least the largest: a near-coincident pair in
+// the sample would dominate the rms).  Needs the gathered sources t.pS.
+void field_scale(Tree& t, hipStream_t st) {
   constexpr int FO = OUT == 4 ? 1 : 0;
-  const int n = (int)t.n, nl = t.nleaves;
-  const int fmode = mode == 4 ? 2 : 1;
-  const int qmin = std::max(PMIN, kMlpOrder0);
-  t.Fp2p.alloc((size_t)std::max(n, 1) * OUT);
-  t.pS.alloc((size_t)std::max(n, 1) * SS);
-  {
-    PhaseTimer pt(t, kTPrep, st);
-    k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
-  }
-  near_field(t, 1, st);
-  // judge sample: FP64 direct sums first; their field scale sets the
-  // tolerance the orders are allocated for
+  const int n = (int)t.n;
   const int J = std::min(kJudgeTargets, n);
   if (t.judge_n != J) {
     std::vector<int> h(J);
@@ -416,47 +408,67 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
     t.judge_n = J;
   }
   t.judge_Fd.alloc((size_t)std::max(J, 1) * OUT);
-  t.judge_acc.alloc(4);
-  t.judge_fail.alloc((size_t)t.ncells + 1);
-  double jscale = 0.0;
-  if (J > 0) {
-    PhaseTimer pt(t, kTJudge, st);
-    k_fill_zero<<<nblk((long long)J * OUT, 256), 256, 0, st>>>((long long)J * OUT, t.judge_Fd.get());
-    const dim3 g(J, (n + kJudgeSlice - 1) / kJudgeSlice);
-    k_judge_direct<<<g, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
-                                      t.judge_Fd.get());
-    std::vector<double> h((size_t)J * OUT);
-    FMM_HIP(hipMemcpyAsync(h.data(), t.judge_Fd.get(), sizeof(double) * J * OUT, hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipStreamSynchronize(st));
-    std::vector<double> m2(J);
-    for (int i = 0; i < J; ++i) {
-      double a = 0.0;
-      for (int k = FO; k < OUT; ++k) a += h[(size_t)i * OUT + k] * h[(size_t)i * OUT + k];
-      m2[i] = a;
-    }
-    // rms |F| of the sample without its largest 0.1% (at least its largest
-    // target: a near-coincident pair in the sample would dominate it)
-    std::sort(m2.begin(), m2.end());
-    const int keep = J - std::max(1, J / 1000);
-    double sum = 0.0;
-    for (int i = 0; i < keep; ++i) sum += m2[i];
-    jscale = keep > 0 ? sqrt(sum / keep) : 0.0;
+  t.judge_scale = 0.0;
+  if (J == 0) return;
+  k_fill_zero<<<nblk((long long)J * OUT, 256), 256, 0, st>>>((long long)J * OUT, t.judge_Fd.get());
+  const dim3 g(J, (n + kJudgeSlice - 1) / kJudgeSlice);
+  k_judge_direct<<<g, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
+                                    t.judge_Fd.get());
+  std::vector<double> h((size_t)J * OUT);
+  FMM_HIP(hipMemcpyAsync(h.data(), t.judge_Fd.get(), sizeof(double) * J * OUT, hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  std::vector<double> m2(J);
+  for (int i = 0; i < J; ++i) {
+    double a = 0.0;
+    for (int k = FO; k < OUT; ++k) a += h[(size_t)i * OUT + k] * h[(size_t)i * OUT + k];
+    m2[i] = a;
   }
+  std::sort(m2.begin(), m2.end());
+  const int keep = J - std::max(1, J / 1000);
+  double sum = 0.0;
+  for (int i = 0; i < keep; ++i) sum += m2[i];
+  t.judge_scale = keep > 0 ? sqrt(sum / keep) : 0.0;
+}
+
+void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
+  FMM_CHECK(t.has_model, kInvalidArgument, "no order model set on the tree");
+  const int n = (int)t.n, nl = t.nleaves;
+  const int fmode = mode == 4 ? 2 : 1;
+  const int qmin = std::max(PMIN, kMlpOrder0);
+  t.Fp2p.alloc((size_t)std::max(n, 1) * OUT);
+  t.pS.alloc((size_t)std::max(n, 1) * SS);
+  if (!t.side) {
+    FMM_HIP(hipStreamCreateWithFlags(&t.side, hipStreamNonBlocking));
+    FMM_HIP(hipEventCreateWithFlags(&t.ev_gather, hipEventDisableTiming));
+    FMM_HIP(hipEventCreateWithFlags(&t.ev_near, hipEventDisableTiming));
+  }
+  {
+    PhaseTimer pt(t, kTPrep, st);
+    k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+  }
+  {
+    PhaseTimer pt(t, kTJudge, st);
+    field_scale(t, st);
+  }
+  const int J = t.judge_n;
   int peff;
-  double tol_eff = t.tol;
   {
     PhaseTimer pt(t, kTOrder, st);
-    leaf_features(t, SS, OUT, FO, st);
-    // predictions are relative to F_near; the target is tol relative to
-    // the field of the judge's sample
-    if (jscale > 0.0 && t.near_scale > 0.0) tol_eff = t.tol * jscale / t.near_scale;
+    leaf_features(t, SS, st);
     t.leaf_q.alloc(std::max(nl, 1));
     t.leaf_y.alloc((size_t)std::max(nl, 1) * kMlpO);
     mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), t.leaf_y.get(), nullptr, 0.f, 0.f, 0, 0, st);
-    t.lstats[7] = allocate_orders(t, t.leaf_y.get(), tol_eff, t.margin, qmin, t.order, st);
+    t.lstats[7] = allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, qmin, t.order, st);
     peff = orders_from_leaves(t, st);
   }
   peff = std::max(std::min(peff, t.order), PMIN);
+  // the near field runs on the side stream beside the far field (the order
+  // selection before it synchronises with the host many times, and the
+  // judge sums compete for the same vector units); L2P waits for it
+  FMM_HIP(hipEventRecord(t.ev_gather, st));
+  FMM_HIP(hipStreamWaitEvent(t.side, t.ev_gather, 0));
+  near_field(t, 1, t.side);
+  FMM_HIP(hipEventRecord(t.ev_near, t.side));
   struct Restore {  // the tile orders apply to this evaluation only
     Tree& t;
     ~Restore() {
@@ -466,11 +478,18 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   } restore{t};
   t.use_tile_order = true;
   t.m2l_classes = &m2l_by_class;
+  bool near_joined = false;
   auto far = [&](int p) {
     OrderTable<PMIN>::far(p, t, fmode, kPhaseAll & ~kPhaseP2P, st);
+    if (!near_joined) {
+      FMM_HIP(hipStreamWaitEvent(st, t.ev_near, 0));
+      near_joined = true;
+    }
     OrderTable<PMIN>::l2p(p, t, fmode, kPhaseAll, F, st);
   };
   far(peff);
+  t.judge_acc.alloc(4);
+  t.judge_fail.alloc((size_t)t.ncells + 1);
   int fails = 0, reruns = 0;
   double jerr = 0.0;
   auto check = [&]() {
@@ -501,7 +520,7 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
     if ((long long)fails * kJudgeDistrust > J) {
       raise_all_leaves(t, t.order, st);
     } else {
-      allocate_orders(t, t.leaf_y.get(), 0.5 * tol_eff, t.margin, qmin, t.order, st);
+      allocate_orders(t, t.leaf_y.get(), 0.5 * t.tol, t.margin, qmin, t.order, st);
       raise_failing_leaves(t, t.order, st);
     }
     peff = std::max(std::min(orders_from_leaves(t, st), t.order), PMIN);
@@ -515,7 +534,7 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   t.lstats[3] = jerr;
   t.lstats[4] = J;
   t.lstats[5] = m2l_mac_fraction(t, t.order, st, &t.lstats[6]);
-  t.lstats[8] = tol_eff / t.tol;
+  t.lstats[8] = t.judge_scale;
   t.lstats[9] = m2l_cell_macs(t, st);
 }
 
@@ -561,6 +580,7 @@ const Variant FMM_VARIANT_NAME = {FMM_VARIANT_NS::SO,
                                   FMM_VARIANT_NS::PMAX,
                                   &FMM_VARIANT_NS::evaluate,
                                   &FMM_VARIANT_NS::evaluate_learned,
+                                  &FMM_VARIANT_NS::field_scale,
                                   &FMM_VARIANT_NS::op_batch,
                                   &FMM_VARIANT_NS::direct};
 #endif

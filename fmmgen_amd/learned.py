@@ -3,8 +3,8 @@
 The order network (three layers, 64 hidden ReLU units, FP32 weights, FP16
 inference; kernels in csrc/mlp.hip) maps twelve features of a leaf cell
 (csrc/learned.h) to the predicted far-field error of the leaf at every
-order 2..12, as log10(rms |dF| over the leaf's particles / F_near), F_near
-the rms near-field magnitude over all targets.  The
+order 2..12, as log10(rms |dF| over the leaf's particles / F_s), F_s the
+field scale of the judge's sample of targets (Tree.field_scale).  The
 learned evaluation (mode "learned") picks the leaf orders with the fewest
 M2L multiply-adds whose predicted norm-wise error (predictions raised by a
 calibrated safety margin) meets the tolerance, runs every M2L tile
@@ -44,11 +44,11 @@ def error_samples(pos, S, ncrit=64, theta=0.5, pmax=None, max_leaves=8000, seed=
     """(X [m, 12], Y [m, 11]) float32 device tensors of one particle set.
 
     Y[:, o] = log10(rms over the leaf's particles of |F_q - F_direct| /
-    F_near) for q = ORDERS[o] (field components), from the plain FP64 FMM
-    at order q and the FP64 direct sum; F_near is the rms near-field (P2P)
-    magnitude without the largest 0.1% (Tree.near_scale), the scale the
-    features use too, known before the far field runs.  Sets with more
-    than ``max_leaves`` leaves contribute a
+    F_s) for q = ORDERS[o] (field components), from the plain FP64 FMM
+    at order q and the FP64 direct sum; F_s is the field scale of the
+    judge sample (Tree.field_scale), the scale the features use too, known
+    before the far field runs.  Sets with more than ``max_leaves`` leaves
+    contribute a
     random sample of that many leaves (the direct sum runs at their
     particles only).
     """
@@ -72,9 +72,9 @@ def error_samples(pos, S, ncrit=64, theta=0.5, pmax=None, max_leaves=8000, seed=
         pick[leaves[rows]] = True
         tg = torch.from_numpy(np.nonzero(pick[leaf_of])[0].astype(np.int64)).cuda()
         Fsub = direct_sum(dpos, dS, tg)
-    tree.compute_field_fmm(F, mode="plain", phases=_lib.PHASE_P2P)
+    tree.compute_field_fmm(F, mode="plain", phases=_lib.PHASE_P2P)  # gathers the sources
     X = tree.leaf_features()[torch.from_numpy(rows).cuda()]
-    fscale = tree.near_scale()
+    fscale = tree.field_scale()
     Y = []
     for q in ORDERS:
         tree.set_order(q)

@@ -81,16 +81,19 @@ class Tree:
         return out
 
     def leaf_features(self):
-        """[nleaves, 12] float32 device tensor of the leaf features (needs a
-        previous evaluation that ran P2P)."""
+        """[nleaves, 12] float32 device tensor of the leaf features (needs
+        one previous evaluation)."""
         f = torch.empty(self.nleaves(), _lib.MLP_FEATURES, dtype=torch.float32, device="cuda")
         _lib.check(_lib.load().fmm_tree_leaf_features(self._h, _ptr(f), _stream()))
         return f
 
-    def near_scale(self):
-        """F_near of the last leaf_features call (the network's output scale)."""
+    def field_scale(self):
+        """Field scale of the learned evaluation (the network's feature and
+        output scale): rms |F| of FP64 direct sums at the judge's 1024
+        stratified targets without the largest 0.1% (needs one previous
+        evaluation)."""
         v = ctypes.c_double()
-        _lib.check(_lib.load().fmm_tree_near_scale(self._h, ctypes.byref(v)))
+        _lib.check(_lib.load().fmm_tree_field_scale(self._h, ctypes.byref(v), _stream()))
         return v.value
 
     def leaf_error(self, F, Fref):
@@ -112,7 +115,7 @@ class Tree:
         a = (ctypes.c_double * 10)()
         _lib.check(_lib.load().fmm_tree_learned_stats(self._h, a))
         keys = ("order_max", "judge_failures", "reruns", "judge_rel_err", "judge_targets", "m2l_mac_fraction",
-                "m2l_macs", "predicted_rel_err", "tol_scale", "m2l_cell_macs")
+                "m2l_macs", "predicted_rel_err", "field_scale", "m2l_cell_macs")
         return dict(zip(keys, list(a)))
 
     def leaf_orders(self):
