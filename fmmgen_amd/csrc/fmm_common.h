@@ -116,7 +116,8 @@ struct M2LSchedule {
   DevBuf<float> Dtab32;
   // (step, lane) -> D slot table of the GEMM plan, uploaded once per order
   static constexpr int kMaxOrder = 16;
-  DevBuf<short> idxtab[kMaxOrder], idxtab32[kMaxOrder];
+  DevBuf<unsigned char> idxtab[kMaxOrder];
+  DevBuf<short> idxtab32[kMaxOrder];
   bool idx_ready[kMaxOrder] = {false}, idx32_ready[kMaxOrder] = {false};
   // learned order: chunks sorted by their tile's order, and the start of
   // every order's range (kMaxOrder + 1 entries)

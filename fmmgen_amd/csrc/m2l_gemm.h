@@ -95,6 +95,48 @@ struct M2LPlan {
   static constexpr int NSTEP = tab.base[NK];
 };
 
+// Multi-index of slot j of the graded layout and its inverse:
+// idx(n) = Nterms(|n| - 1) + T(|n| - n_x) + (|n| - n_x - n_y), T(a) = a(a+1)/2.
+struct MultiIdx {
+  int x, y, z;
+};
+__host__ __device__ constexpr MultiIdx slot_multi(int j) {
+  const int d = slot_degree(j);
+  const int r = j - nterms(d - 1);
+  int a = 0;
+  while ((a + 1) * (a + 2) / 2 <= r) ++a;
+  const int b = r - a * (a + 1) / 2;
+  return {d - a, a - b, b};
+}
+__host__ __device__ constexpr int multi_slot(int x, int y, int z) {
+  const int d = x + y + z, a = d - x;
+  return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
+}
+
+// D-vector slot (full index - OFF) lane l reads at step (I, K): the entry
+// T[16 I + l%16][4 K + l/16] = D[n + m], or -1 outside the expansion.
+template <int P, int SO>
+__host__ __device__ constexpr int step_lane_slot(int I, int K, int l) {
+  constexpr int LS = lsize(P, SO), MS = msize(P, SO), OFF = nterms(SO - 1);
+  const int n = 16 * I + (l & 15), m = 4 * K + (l >> 4);
+  if (n >= LS || m >= MS) return -1;
+  const MultiIdx a = slot_multi(n), b = slot_multi(m + OFF);
+  if (a.x + a.y + a.z + b.x + b.y + b.z > P) return -1;
+  return multi_slot(a.x + b.x, a.y + b.y, a.z + b.z) - OFF;
+}
+
+// Lowest D slot a step reads: the index table holds 8-bit offsets from it
+// (255 marks an empty entry), the base is a compile-time immediate.
+template <int P, int SO>
+__host__ __device__ constexpr int step_base(int I, int K) {
+  int b = 1 << 30;
+  for (int l = 0; l < 64; ++l) {
+    const int v = step_lane_slot<P, SO>(I, K, l);
+    if (v >= 0 && v < b) b = v;
+  }
+  return b == (1 << 30) ? 0 : b;
+}
+
 template <typename T>
 struct Vec4;
 template <>
@@ -159,7 +201,7 @@ struct WavePlan {
 template <int P, int SO, typename T>
 constexpr int m2l_waves() {
   using Plan = M2LPlan<P, SO>;
-  constexpr int lds = Plan::NSTEP * 64 * 2 + (int)sizeof(T) * (kTile * (Plan::MS | 1) + Plan::MS);
+  constexpr int lds = Plan::NSTEP * 64 + (int)sizeof(T) * (kTile * (Plan::MS | 1) + Plan::MS + 256);
   return lds > 80 * 1024 ? 8 : 4;
 }
 
@@ -183,7 +225,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
                                                      const int* __restrict__ group_key,
                                                      const int* __restrict__ group_src,
                                                      const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
-                                                     const short* __restrict__ idxtab,
+                                                     const unsigned char* __restrict__ idxtab,
                                                      const signed char* __restrict__ tile_order,
                                                      const int* __restrict__ chunk_list, int mstride, int lstride,
                                                      const T* __restrict__ M, T* __restrict__ L) {
@@ -194,20 +236,17 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   constexpr int CPW = kTile / W;         // columns staged by each wave
   constexpr int DJ = (NJ + W - 1) / W;   // D segments staged by each wave
   using V4 = typename Vec4<T>::type;
-  __shared__ short sidx[NST * 64];
-  __shared__ T sD[MS + 1];                // slot MS stays zero (empty A entries)
-  __shared__ T sB[kTile * MSP + 4];       // tail slots stay zero
+  __shared__ unsigned char sidx[NST * 64];  // offsets from each step's base slot
+  __shared__ T sD[MS + 256];                 // slots MS .. MS + 255 stay zero
+  __shared__ T sB[kTile * MSP + 4];          // tail slots stay zero
   __shared__ signed char sdeg[MS];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int ci = xcd_tile(blockIdx.x, nblocks);
   if (ci >= nchunks) return;  // uniform over the block
   const int chunk = chunk_list ? chunk_list[ci] : ci;
   const int tile = chunk_tile[chunk];
-  for (int i = tid; i < NST * 64; i += 64 * W) {
-    const short v = idxtab[i];
-    sidx[i] = v < 0 ? (short)MS : v;
-  }
-  if (tid == 0) sD[MS] = T(0);
+  for (int i = tid; i < NST * 64; i += 64 * W) sidx[i] = idxtab[i];
+  for (int i = tid; i < 256; i += 64 * W) sD[MS + i] = T(0);
   for (int i = tid; i < kTile * MSP + 4; i += 64 * W) sB[i] = T(0);
   for (int j = tid; j < MS; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
   const int col = lane & 15, kk = lane >> 4;
@@ -281,8 +320,13 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
             constexpr int i = bb * kBatch + u;
             if constexpr (i < NSW) {
               constexpr int s = WP::tab.s[i];
-              constexpr int K = Plan::tab.sK[s];
-              av[u] = sD[sidx[s * 64 + lane]];
+              constexpr int K = Plan::tab.sK[s], I = Plan::tab.sI[s];
+              constexpr int base = step_base<P, SO>(I, K);
+              const int off = sidx[s * 64 + lane];
+              av[u] = sD[base + off];
+              // an empty entry (offset 255) reads a zero slot when the
+              // step's base is within 255 of MS, else it is masked here
+              if constexpr (MS - base > 255) av[u] = off == 255 ? T(0) : av[u];
               bv[u] = sB[col * MSP + 4 * K + kk];
             }
           });
@@ -531,37 +575,22 @@ std::vector<short> m2l_index_table32() {
   return t;
 }
 
-// Host side: the (step, lane) -> D slot table of the plan (-1: zero entry).
+// Host side: the (step, lane) table of the plan: 8-bit offsets of the D
+// slot from the step's base slot, 255 for an empty entry.
 template <int P, int SO>
-std::vector<short> m2l_index_table() {
+std::vector<unsigned char> m2l_index_table() {
   using Plan = M2LPlan<P, SO>;
-  // multi-index of every slot up to degree 2p
-  std::vector<int> mx, my, mz;
-  for (int d = 0; d <= 2 * P; ++d)
-    for (int x = d; x >= 0; --x)
-      for (int y = d - x; y >= 0; --y) {
-        mx.push_back(x);
-        my.push_back(y);
-        mz.push_back(d - x - y);
-      }
-  auto idx = [](int x, int y, int z) {
-    const int d = x + y + z;
-    const int a = d - x;
-    return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
-  };
-  std::vector<short> t((size_t)Plan::NSTEP * 64, -1);
-  for (int K = 0; K < Plan::NK; ++K)
-    for (int I = 0; I < Plan::tab.nI[K]; ++I) {
-      const int s = Plan::tab.base[K] + I;
-      for (int l = 0; l < 64; ++l) {
-        const int n = 16 * I + (l & 15), m = 4 * K + (l >> 4);
-        if (n >= Plan::LS || m >= Plan::MS) continue;
-        const int mf = m + Plan::OFF;  // full index of the multipole slot
-        const int dn = mx[n] + my[n] + mz[n], dm = mx[mf] + my[mf] + mz[mf];
-        if (dn + dm > P) continue;
-        t[(size_t)s * 64 + l] = (short)(idx(mx[n] + mx[mf], my[n] + my[mf], mz[n] + mz[mf]) - Plan::OFF);
-      }
+  std::vector<unsigned char> t((size_t)Plan::NSTEP * 64, 255);
+  for (int s = 0; s < Plan::NSTEP; ++s) {
+    const int I = Plan::tab.sI[s], K = Plan::tab.sK[s];
+    const int base = step_base<P, SO>(I, K);
+    for (int l = 0; l < 64; ++l) {
+      const int v = step_lane_slot<P, SO>(I, K, l);
+      if (v < 0) continue;
+      if (v - base > 254) throw Error{kUnsupported, "M2L step spans more than 255 D slots"};
+      t[(size_t)s * 64 + l] = (unsigned char)(v - base);
     }
+  }
   return t;
 }
 

@@ -117,12 +117,12 @@ __global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const doubl
 }
 
 template <int P>
-const short* ensure_index_table(Tree& t, hipStream_t st) {
+const unsigned char* ensure_index_table(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (!S.idx_ready[P]) {
-    const std::vector<short> tab = m2l_index_table<P, SO>();
+    const std::vector<unsigned char> tab = m2l_index_table<P, SO>();
     S.idxtab[P].alloc(tab.size());
-    FMM_HIP(hipMemcpyAsync(S.idxtab[P].get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
+    FMM_HIP(hipMemcpyAsync(S.idxtab[P].get(), tab.data(), tab.size(), hipMemcpyHostToDevice, st));
     FMM_HIP(hipStreamSynchronize(st));
     S.idx_ready[P] = true;
   }
@@ -146,7 +146,7 @@ template <int P>
 void m2l_plain(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (S.ngroups == 0) return;
-  const short* idx = ensure_index_table<P>(t, st);
+  const unsigned char* idx = ensure_index_table<P>(t, st);
   S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
   const int grid = ((S.nchunks + 7) / 8) * 8;
@@ -254,7 +254,7 @@ void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hi
                                              ms, ls, t.Mf.get(), t.Lf.get());
     return;
   }
-  const short* idx = ensure_index_table<Q>(t, st);
+  const unsigned char* idx = ensure_index_table<Q>(t, st);
   constexpr int W = m2l_waves<Q, SO, float>();
   k_m2l_gemm<Q, SO, float, true, W><<<grid, 64 * W, 0, st>>>(
       nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
