@@ -148,9 +148,6 @@ int fmm_tree_destroy(fmm_tree_t tree) {
         (void)hipEventDestroy(e.second.second);
       }
       for (auto e : tree->t.event_pool) (void)hipEventDestroy(e);
-      if (tree->t.ev_gather) (void)hipEventDestroy(tree->t.ev_gather);
-      if (tree->t.ev_near) (void)hipEventDestroy(tree->t.ev_near);
-      if (tree->t.side) (void)hipStreamDestroy(tree->t.side);
     }
     delete tree;
   });

@@ -124,6 +124,8 @@ struct M2LSchedule {
   DevBuf<int> class_chunks;
   DevBuf<long long> tile_pairs;  // M2L pairs per tile (statistics)
   bool tile_pairs_ready = false;
+  DevBuf<int> cell_tile;         // cell -> its M2L tile (-1: none)
+  bool cell_tile_ready = false;
   // work items: chunks of at most kGroupsPerChunk groups of one tile
   int nchunks = 0;
   DevBuf<int> chunk_tile, chunk_g0, chunk_g1;
@@ -181,14 +183,12 @@ struct Tree {
   // order of its 16 target cells; the tile's groups are truncated there)
   DevBuf<float> leaf_feat;         // [nleaves * kMlpF]
   DevBuf<int> leaf_q, cell_q;      // [nleaves], [ncells]
+  DevBuf<int> cell_eff;            // [ncells] effective order (lowest tile order on the chain)
   DevBuf<float> leaf_y;            // [nleaves * kMlpO] network outputs
   DevBuf<double> cell_qabs, cell_q2;  // [ncells] sum of |S| and of |S|^2 below the cell
   DevBuf<double> cell_acc;            // [ncells * 11] list sums, accumulated down the tree
   double judge_scale = 0.0;        // field scale of the judge sample (network output scale)
-  // side stream of the learned evaluation (P2P runs beside order selection
-  // and the far field) and its two events; created on first use
-  hipStream_t side = nullptr;
-  hipEvent_t ev_gather = nullptr, ev_near = nullptr;
+
   DevBuf<signed char> tile_order;  // [ntiles]
   bool use_tile_order = false;
   const signed char* tile_order_ptr() const { return use_tile_order ? tile_order.get() : nullptr; }

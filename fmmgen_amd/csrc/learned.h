@@ -47,6 +47,11 @@ void raise_failing_leaves(Tree& t, int qfull, hipStream_t st);
 // start[q] .. start[q + 1] is order q's range (start has
 // M2LSchedule::kMaxOrder + 1 entries, host).
 void chunk_classes(Tree& t, hipStream_t st, int* start);
+// Predicted norm-wise error relative to F_s of the current tile orders: each
+// leaf at its effective order, the lowest tile order on its ancestor chain
+// (at least its own: a tile runs at the largest order of its cells), so the
+// allocation can spend the accuracy the tile maxima add.
+double effective_error(Tree& t, const float* Y, double margin, int qmax, hipStream_t st);
 // Sum over M2L pairs of C(q + 6, 6) at the order of the pair's target
 // cell (what the truncated M2L would cost without the tile maximum).
 double m2l_cell_macs(Tree& t, hipStream_t st);

@@ -295,7 +295,78 @@ __global__ void k_cell_macs(int ncells, const int* __restrict__ off, const int* 
   if (threadIdx.x == 0) atomicAdd(out, red[0]);
 }
 
+// cell -> M2L tile of the cell (-1: no M2L targets), once per tree
+__global__ void k_cell_tile(int ntiles, const int* __restrict__ tile_cells, int* __restrict__ cell_tile) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntiles * kTile) return;
+  const int c = tile_cells[i];
+  if (c >= 0) cell_tile[c] = i / kTile;
+}
+
+// effective order of every cell: the lowest tile order over its ancestor
+// chain (the M2L orders its particles' far field actually ran at), one
+// level at a time from the root
+__global__ void k_eff_down(int beg, int end, const int* __restrict__ parent, const int* __restrict__ cell_tile,
+                           const signed char* __restrict__ tile_order, int qmax, int* __restrict__ eff) {
+  const int c = beg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= end) return;
+  int q = c == 0 ? qmax : eff[parent[c]];
+  const int tl = cell_tile[c];
+  if (tl >= 0) q = min(q, (int)tile_order[tl]);
+  eff[c] = q;
+}
+
+// sum over leaves of n_l e_l(q_eff)^2 with e = 10^(Y + margin)
+__global__ void k_eff_err(int nl, const int* __restrict__ leaves, const float* __restrict__ feat,
+                          const float* __restrict__ Y, const int* __restrict__ eff, float margin,
+                          double* __restrict__ out) {
+  __shared__ double red[256];
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (li < nl) {
+    int o = eff[leaves[li]] - kMlpOrder0;
+    o = o < 0 ? 0 : (o >= kMlpO ? kMlpO - 1 : o);
+    const double e2 = exp10(2.0 * ((double)Y[(size_t)li * kMlpO + o] + margin));
+    v = exp2((double)feat[(size_t)li * kMlpF + 6]) * e2;
+  }
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
 }  // namespace
+
+double effective_error(Tree& t, const float* Y, double margin, int qmax, hipStream_t st) {
+  const int nc = t.ncells, nl = t.nleaves, nt = t.sched.ntiles;
+  if (nl == 0) return 0.0;
+  if (!t.sched.cell_tile_ready) {
+    t.sched.cell_tile.alloc(nc);
+    k_fill_i<<<nb(nc, 256), 256, 0, st>>>(nc, t.sched.cell_tile.get(), -1);
+    if (nt > 0)
+      k_cell_tile<<<nb((long long)nt * kTile, 256), 256, 0, st>>>(nt, t.sched.tile_cells.get(),
+                                                                 t.sched.cell_tile.get());
+    t.sched.cell_tile_ready = true;
+  }
+  t.cell_eff.alloc(nc);
+  for (int l = 0; l < t.depth; ++l) {
+    const int b = t.level_off[l], e = t.level_off[l + 1];
+    k_eff_down<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.parent.get(), t.sched.cell_tile.get(), t.tile_order.get(),
+                                               qmax, t.cell_eff.get());
+  }
+  DevBuf<double> acc;
+  acc.alloc(1);
+  k_zero_d<<<1, 64, 0, st>>>(1, acc.get());
+  k_eff_err<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaves.get(), t.leaf_feat.get(), Y, t.cell_eff.get(), (float)margin,
+                                         acc.get());
+  double h = 0.0;
+  FMM_HIP(hipMemcpyAsync(&h, acc.get(), sizeof(double), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  return sqrt(h / (double)t.n);
+}
 
 double m2l_cell_macs(Tree& t, hipStream_t st) {
   const int nc = t.ncells;

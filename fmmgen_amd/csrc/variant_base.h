@@ -367,9 +367,9 @@ void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
 // Learned-order evaluation: the judge sample's FP64 direct sums and field
 // scale, leaf features, FP16 network inference -> leaf orders (the
 // cheapest whose predicted norm-wise error meets the tolerance) -> cell
-// and tile orders, mixed-precision far field at the largest order with
-// every M2L tile truncated at its own order (FP32 P2P on a side stream
-// meanwhile), then the judge's check at its kJudgeTargets targets.  When the sample's relative field error exceeds the tolerance,
+// and tile orders, FP32 P2P, mixed-precision far field at the largest
+// order with every M2L tile truncated at its own order, then the judge's
+// check at its kJudgeTargets targets.  When the sample's relative field error exceeds the tolerance,
 // the leaves holding a target whose error exceeds tol * rms |F| rerun at
 // the tree's order and the other leaves are re-allocated for half the
 // tolerance (all leaves run at the tree's order when more than one target
@@ -437,11 +437,6 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   const int qmin = std::max(PMIN, kMlpOrder0);
   t.Fp2p.alloc((size_t)std::max(n, 1) * OUT);
   t.pS.alloc((size_t)std::max(n, 1) * SS);
-  if (!t.side) {
-    FMM_HIP(hipStreamCreateWithFlags(&t.side, hipStreamNonBlocking));
-    FMM_HIP(hipEventCreateWithFlags(&t.ev_gather, hipEventDisableTiming));
-    FMM_HIP(hipEventCreateWithFlags(&t.ev_near, hipEventDisableTiming));
-  }
   {
     PhaseTimer pt(t, kTPrep, st);
     k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
@@ -458,17 +453,31 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
     t.leaf_q.alloc(std::max(nl, 1));
     t.leaf_y.alloc((size_t)std::max(nl, 1) * kMlpO);
     mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), t.leaf_y.get(), nullptr, 0.f, 0.f, 0, 0, st);
-    t.lstats[7] = allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, qmin, t.order, st);
+    allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, qmin, t.order, st);
     peff = orders_from_leaves(t, st);
+    // the tile maxima leave the predicted error below the budget: allocate
+    // once more for the tolerance scaled by that slack, keep it when its
+    // effective error still meets the tolerance
+    double e1 = effective_error(t, t.leaf_y.get(), t.margin, t.order, st);
+    if (e1 > 0.0 && e1 < 0.9 * t.tol) {
+      const double tol2 = std::min(t.tol * t.tol / e1, 4.0 * t.tol);
+      allocate_orders(t, t.leaf_y.get(), tol2, t.margin, qmin, t.order, st);
+      const int p2 = orders_from_leaves(t, st);
+      const double e2 = effective_error(t, t.leaf_y.get(), t.margin, t.order, st);
+      if (e2 <= t.tol) {
+        peff = p2;
+        e1 = e2;
+      } else {
+        allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, qmin, t.order, st);
+        peff = orders_from_leaves(t, st);
+      }
+    }
+    t.lstats[7] = e1;
   }
   peff = std::max(std::min(peff, t.order), PMIN);
-  // the near field runs on the side stream beside the far field (the order
-  // selection before it synchronises with the host many times, and the
-  // judge sums compete for the same vector units); L2P waits for it
-  FMM_HIP(hipEventRecord(t.ev_gather, st));
-  FMM_HIP(hipStreamWaitEvent(t.side, t.ev_gather, 0));
-  near_field(t, 1, t.side);
-  FMM_HIP(hipEventRecord(t.ev_near, t.side));
+  // near field (FP32, compensated); running it on a second stream beside
+  // the M2L was measured to gain nothing: both keep the compute units busy
+  near_field(t, 1, st);
   struct Restore {  // the tile orders apply to this evaluation only
     Tree& t;
     ~Restore() {
@@ -478,13 +487,8 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   } restore{t};
   t.use_tile_order = true;
   t.m2l_classes = &m2l_by_class;
-  bool near_joined = false;
   auto far = [&](int p) {
     OrderTable<PMIN>::far(p, t, fmode, kPhaseAll & ~kPhaseP2P, st);
-    if (!near_joined) {
-      FMM_HIP(hipStreamWaitEvent(st, t.ev_near, 0));
-      near_joined = true;
-    }
     OrderTable<PMIN>::l2p(p, t, fmode, kPhaseAll, F, st);
   };
   far(peff);
