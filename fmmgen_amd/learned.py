@@ -88,7 +88,7 @@ def error_samples(pos, S, ncrit=64, theta=0.5, pmax=None, max_leaves=8000, seed=
 
 
 def problem_set(total, seed=0, nmin=1 << 16, nmax=1 << 22):
-    """Particle sets of mixed kinds and sizes until ``total`` leaves (samples) are reached."""
+    """Particle sets of mixed kinds, sizes and leaf capacities (ncrit)."""
     rng = np.random.default_rng(seed)
     kinds = ["blobs", "blobs", "uniform", "plummer", "mixed", "blobs_tight", "blobs_wide"]
     k = 0
@@ -103,7 +103,8 @@ def problem_set(total, seed=0, nmin=1 << 16, nmax=1 << 22):
                                   frac_bg=float(rng.uniform(0.0, 0.5)))
         else:
             pos = workloads.positions(kind, n, s)
-        yield kind, pos, workloads.charges(n, s + 1)
+        ncrit = int(rng.choice([32, 64, 64, 128]))
+        yield kind, pos, workloads.charges(n, s + 1), ncrit
         k += 1
 
 
@@ -111,13 +112,13 @@ def make_dataset(total, seed=0, log=print, **kw):
     """(X, Y, problem id per sample) over particle sets until ``total`` samples."""
     Xs, Ys, Ps, got = [], [], [], 0
     t0 = time.time()
-    for k, (kind, pos, S) in enumerate(problem_set(total, seed, **kw)):
-        X, Y = error_samples(pos, S, seed=seed + k)
+    for k, (kind, pos, S, ncrit) in enumerate(problem_set(total, seed, **kw)):
+        X, Y = error_samples(pos, S, ncrit=ncrit, seed=seed + k)
         Xs.append(X)
         Ys.append(Y)
         Ps.append(torch.full((X.shape[0],), k, dtype=torch.int32, device=X.device))
         got += X.shape[0]
-        log(f"  data: {kind:12s} n={pos.shape[0]:7d} leaves={X.shape[0]:6d} total={got:7d} "
+        log(f"  data: {kind:12s} n={pos.shape[0]:7d} ncrit={ncrit:3d} leaves={X.shape[0]:6d} total={got:7d} "
             f"({time.time() - t0:.1f}s)")
         if got >= total:
             break
