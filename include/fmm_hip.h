@@ -143,9 +143,9 @@ int fmm_export_expansions(fmm_tree_t tree, double* h_M, double* h_L);
  * Network: FMM_MLP_FEATURES inputs, two hidden ReLU layers of
  * FMM_MLP_HIDDEN units, FMM_MLP_OUTPUTS outputs; FP32 parameters, flat
  * [W1 (H x F), b1 (H), W2 (H x H), b2 (H), W3 (O x H), b3 (O)], row-major.
- * Output o predicts log10(rms |dF| over a leaf's particles / F_near) of
- * the far field at order FMM_MLP_ORDER0 + o, F_near the rms near-field
- * (P2P) magnitude (fmm_tree_near_scale).  All pointers are device
+ * Output o predicts log10(rms |dF| over a leaf's particles / F_s) of the
+ * far field at order FMM_MLP_ORDER0 + o, F_s the field scale
+ * (fmm_tree_field_scale).  All pointers are device
  * pointers except where named h_. */
 #define FMM_MLP_FEATURES 12
 #define FMM_MLP_HIDDEN 64
@@ -175,12 +175,14 @@ int fmm_mlp_adam(const float* d_grad, float* d_params, float* d_m, float* d_v, f
  * do not depend on it); the learned modes use it as the full order. */
 int fmm_tree_set_order(fmm_tree_t tree, int order);
 /* Leaf features d_feat [nleaves*FMM_MLP_FEATURES], leaves in canonical
- * order; they use the near field of the last evaluation that ran P2P. */
+ * order (computes the field scale first; needs one previous evaluation). */
 int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream);
-/* F_near of the last feature pass (rms near-field magnitude over the
- * targets outside the largest 0.1% of magnitudes), the scale of the
- * network's outputs: h_scale [1]. */
-int fmm_tree_near_scale(fmm_tree_t tree, double* h_scale);
+/* Field scale F_s of the learned evaluation, the scale of the network's
+ * features and outputs: the rms |F| (field components) of FP64 direct sums
+ * at 1024 stratified targets (sorted slots floor((2i + 1) n / 2048)),
+ * without the largest 0.1% of them (at least the largest).  Needs one
+ * previous evaluation (for the gathered sources): h_scale [1]. */
+int fmm_tree_field_scale(fmm_tree_t tree, double* h_scale, void* stream);
 /* Root-mean-square |F - Fref| (field components) over each leaf's
  * particles: d_err [nleaves]; d_F, d_Fref [n*OUTPUTSIZE] in caller order. */
 int fmm_tree_leaf_error(fmm_tree_t tree, const double* d_F, const double* d_Fref, double* d_err, void* stream);
@@ -195,8 +197,8 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
  * error (after the rerun, if any), judge targets, M2L multiply-adds
  * relative to the tree's order, M2L useful multiply-adds (sum over pairs of
  * C(q + 6, 6) at the pair's order), predicted error of the chosen orders
- * relative to F_near, the factor applied to tol for the allocation (the
- * judge sample's field scale over F_near), 0. */
+ * relative to F_s, the field scale F_s, M2L multiply-adds at the target
+ * cells' own orders (without the tile maximum). */
 int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats);
 /* Per-leaf orders of the last learned evaluation: h_order [nleaves]. */
 int fmm_tree_leaf_orders(fmm_tree_t tree, int* h_order);

@@ -143,7 +143,7 @@ def _cpu_features(tree, q, Fnear):
         acc[c, :9] += acc[p, :9]
         acc[c, 9] = max(acc[c, 9], acc[p, 9])
         acc[c, 10] += acc[p, 10]
-    fn = tree.near_scale()
+    fn = tree.field_scale()
     leaves = tree.leaves()
     a = acc[leaves]
     lg = lambda v: np.clip(np.log10(np.maximum(v, 1e-300) / fn), -40, 20)
@@ -168,11 +168,14 @@ def test_leaf_features_and_errors_match_numpy(gpu):
     fg = t.leaf_features().cpu().numpy()
     fc = _cpu_features(t, q, Fn.cpu().numpy())
     assert np.abs(fg - fc).max() <= 1e-4 * max(1.0, np.abs(fc).max())
-    # the scale: rms |F_near| without the largest 0.1% (histogram bins of 2^(1/4))
-    m = np.sort(np.sqrt((Fn.cpu().numpy()[:, 1:] ** 2).sum(1)))
-    lo = np.sqrt((m[: n - int(0.001 * n)] ** 2).mean())
-    hi = np.sqrt((m ** 2).mean())
-    assert lo / 1.2 <= t.near_scale() <= hi
+    # the scale: FP64 direct sums at the 1024 stratified sorted slots, rms
+    # |F| without the largest
+    from fmmgen_amd import direct_sum
+    perm, _ = t.export_particles()
+    slots = ((2 * np.arange(1024) + 1) * n) // 2048
+    Fj = direct_sum(_dev(pos), _dev(q), torch.from_numpy(perm[slots]).cuda()).cpu().numpy()
+    m2 = np.sort((Fj[:, 1:] ** 2).sum(1))[:-1]
+    assert abs(t.field_scale() - np.sqrt(m2.mean())) <= 1e-12 * np.sqrt(m2.mean())
     # per-leaf rms error
     F = torch.zeros_like(Fn)
     t.compute_field_fmm(F, mode="plain")
