@@ -1,5 +1,6 @@
 // Features, order propagation and bookkeeping of the learned order
 // (semantics in learned.h).
+#include <algorithm>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -338,6 +339,45 @@ __global__ void k_eff_err(int nl, const int* __restrict__ leaves, const float* _
   if (threadIdx.x == 0) atomicAdd(out, red[0]);
 }
 
+// The error sums of kLam multipliers at once: lam_k = 10^(lo + k step).
+constexpr int kLam = 64;
+__global__ void k_alloc_multi(int nl, const float* __restrict__ feat, const float* __restrict__ Y, double lo,
+                              double step, float margin, int qmin, int qmax, double* __restrict__ sums) {
+  __shared__ float sh[kLam];
+  for (int k = threadIdx.x; k < kLam; k += blockDim.x) sh[k] = 0.f;
+  __syncthreads();
+  const int li = blockIdx.x * blockDim.x + threadIdx.x;
+  if (li < nl) {
+    const float* f = feat + (size_t)li * kMlpF;
+    const double w = fmax(exp10((double)f[5]) - 1.0, 1.0);
+    const double n = exp2((double)f[6]);
+    double c[kMlpO], e2[kMlpO];
+    for (int o = 0; o < kMlpO; ++o) {
+      const int q = o + kMlpOrder0;
+      const bool ok = q >= qmin && q <= qmax;
+      c[o] = ok ? macs(q) * w : -1.0;
+      e2[o] = exp10(2.0 * ((double)Y[(size_t)li * kMlpO + o] + margin));
+    }
+    for (int k = 0; k < kLam; ++k) {
+      const double lam = exp10(lo + k * step) * n;
+      double best = 0.0, be = 0.0;
+      bool any = false;
+      for (int o = 0; o < kMlpO; ++o) {
+        if (c[o] < 0.0) continue;
+        const double v = c[o] + lam * e2[o];
+        if (!any || v < best) {
+          best = v;
+          be = e2[o];
+          any = true;
+        }
+      }
+      atomicAdd(&sh[k], (float)(n * be));
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < kLam; k += blockDim.x) atomicAdd(&sums[k], (double)sh[k]);
+}
+
 }  // namespace
 
 double effective_error(Tree& t, const float* Y, double margin, int qmax, hipStream_t st) {
@@ -384,27 +424,45 @@ double allocate_orders(Tree& t, const float* Y, double tol, double margin, int q
   const int nl = t.nleaves;
   if (nl == 0) return 0.0;
   DevBuf<double> acc;
-  acc.alloc(1);
+  acc.alloc(kLam);
   const double target = tol * tol * (double)t.n;
-  auto eval = [&](double loglam) {
-    k_zero_d<<<1, 64, 0, st>>>(1, acc.get());
-    k_alloc<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, loglam, (float)margin, qmin, qmax,
-                                         t.leaf_q.get(), acc.get());
-    double h = 0.0;
-    FMM_HIP(hipMemcpyAsync(&h, acc.get(), sizeof(double), hipMemcpyDeviceToHost, st));
+  // two passes of kLam multipliers (a coarse grid over 10^-10 .. 10^40, then
+  // a fine grid inside the bracket): the smallest multiplier whose orders
+  // meet the target
+  auto sweep = [&](double lo, double step, std::vector<double>& out) {
+    k_zero_d<<<1, kLam, 0, st>>>(kLam, acc.get());
+    k_alloc_multi<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, lo, step, (float)margin, qmin, qmax,
+                                               acc.get());
+    out.resize(kLam);
+    FMM_HIP(hipMemcpyAsync(out.data(), acc.get(), sizeof(double) * kLam, hipMemcpyDeviceToHost, st));
     FMM_HIP(hipStreamSynchronize(st));
-    return h;
   };
-  double lo = -10.0, hi = 40.0;
-  double e = eval(hi);
-  if (e > target) return sqrt(e / (double)t.n);  // even the highest orders miss: best effort
-  for (int it = 0; it < 34; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (eval(mid) <= target) hi = mid;
-    else lo = mid;
+  std::vector<double> e;
+  double lo = -10.0, step = 50.0 / (kLam - 1);
+  sweep(lo, step, e);
+  int k = 0;
+  while (k < kLam && e[k] > target) ++k;
+  double loglam;
+  if (k == kLam) {
+    loglam = lo + (kLam - 1) * step;  // even the highest orders miss: best effort
+  } else if (k == 0) {
+    loglam = lo;
+  } else {
+    const double flo = lo + (k - 1) * step;
+    const double fstep = step / (kLam - 1);
+    sweep(flo, fstep, e);
+    int j = 0;
+    while (j < kLam && e[j] > target) ++j;
+    loglam = flo + std::min(j, kLam - 1) * fstep;
   }
-  e = eval(hi);
-  return sqrt(e / (double)t.n);
+  // the orders of the chosen multiplier
+  k_zero_d<<<1, 64, 0, st>>>(1, acc.get());
+  k_alloc<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, loglam, (float)margin, qmin, qmax, t.leaf_q.get(),
+                                       acc.get());
+  double h = 0.0;
+  FMM_HIP(hipMemcpyAsync(&h, acc.get(), sizeof(double), hipMemcpyDeviceToHost, st));
+  FMM_HIP(hipStreamSynchronize(st));
+  return sqrt(h / (double)t.n);
 }
 
 void chunk_classes(Tree& t, hipStream_t st, int* start) {

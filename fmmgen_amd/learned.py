@@ -221,29 +221,33 @@ def allocate(Yp, counts, w, tol, margin, qmin=ORDER0, qmax=ORDERS[-1]):
     """Column (order - ORDER0) per leaf chosen as the device does
     (csrc/learned.hip, allocate_orders): the cheapest orders, by
     C(q + 6, 6) w_l + lam n_l e_l(q)^2, whose predicted norm-wise error
-    meets tol, lam found by bisection on log10 lam in [-10, 40]."""
+    meets tol; log10 lam is the first point meeting it on a grid of 64 over
+    [-10, 40], refined by a grid of 64 inside the bracket."""
     from math import comb
     cols = np.array([o for o, q in enumerate(ORDERS) if qmin <= q <= qmax])
     macs = np.array([comb(ORDERS[o] + 6, 6) for o in cols], dtype=np.float64)
     e2 = 10.0 ** (2.0 * (Yp[:, cols] + margin))
     target = tol * tol * counts.sum()
+    K = 64
 
     def pick(loglam):
         c = macs[None, :] * w[:, None] + 10.0 ** loglam * counts[:, None] * e2
         o = c.argmin(1)
         return o, (counts * e2[np.arange(len(o)), o]).sum()
 
-    lo, hi = -10.0, 40.0
-    o, e = pick(hi)
-    if e > target:
-        return cols[o]
-    for _ in range(34):
-        mid = 0.5 * (lo + hi)
-        if pick(mid)[1] <= target:
-            hi = mid
-        else:
-            lo = mid
-    return cols[pick(hi)[0]]
+    lo, step = -10.0, 50.0 / (K - 1)
+    errs = [pick(lo + k * step)[1] for k in range(K)]
+    k = next((i for i, e in enumerate(errs) if e <= target), K)
+    if k == K:
+        loglam = lo + (K - 1) * step
+    elif k == 0:
+        loglam = lo
+    else:
+        flo, fstep = lo + (k - 1) * step, step / (K - 1)
+        errs = [pick(flo + j * fstep)[1] for j in range(K)]
+        j = next((i for i, e in enumerate(errs) if e <= target), K - 1)
+        loglam = flo + j * fstep
+    return cols[pick(loglam)[0]]
 
 
 def problem_errors(Y_true, Y_pred, X, pid, tol, margin):
