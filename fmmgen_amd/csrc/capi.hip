@@ -443,6 +443,17 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
     t.has_model = true;
     t.tol = tol;
     t.margin = margin;
+    // re-form the M2L tiles with the predicted cell orders as a key inside
+    // each class, so a tile's cells share their order
+    if (t.so == 0 && t.sched.ntiles > 0 && t.n > 0) {
+      tree->v->predict_cell_orders(t, nullptr);
+      t.tile_hint.alloc(t.ncells);
+      FMM_HIP(hipMemcpyAsync(t.tile_hint.get(), t.cell_q.get(), sizeof(int) * t.ncells, hipMemcpyDeviceToDevice,
+                             nullptr));
+      t.use_tile_hint = true;
+      build_m2l_schedule(t, nullptr);
+      FMM_HIP(hipStreamSynchronize(nullptr));
+    }
   });
 }
 

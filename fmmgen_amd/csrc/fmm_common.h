@@ -184,6 +184,9 @@ struct Tree {
   DevBuf<float> leaf_feat;         // [nleaves * kMlpF]
   DevBuf<int> leaf_q, cell_q;      // [nleaves], [ncells]
   DevBuf<int> cell_eff;            // [ncells] effective order (lowest tile order on the chain)
+  DevBuf<int> tile_hint;           // [ncells] order hint of the M2L tiling (set with a model)
+  bool use_tile_hint = false;
+  const int* tile_hint_ptr() const { return use_tile_hint ? tile_hint.get() : nullptr; }
   DevBuf<float> leaf_y;            // [nleaves * kMlpO] network outputs
   DevBuf<double> cell_qabs, cell_q2;  // [ncells] sum of |S| and of |S|^2 below the cell
   DevBuf<double> cell_acc;            // [ncells * 11] list sums, accumulated down the tree

@@ -38,14 +38,18 @@ __global__ void k_has_m2l(int b, int e, const int* __restrict__ off, int* __rest
 // canonical order inside a class.  Away from the domain boundary, cells of
 // one class see the same set of relative M2L offsets, so a tile of them
 // fills its 16 columns for most offsets.  Cells without M2L work sort last.
+// With a per-cell order hint (the learned order the cell is predicted to
+// run at), cells of one class are further ordered by hint, so that a tile
+// (which runs at the largest order of its cells) holds cells of one order.
 __global__ void k_tile_keys(int b, int e, const int* __restrict__ off, const int* __restrict__ octant,
-                            const int* __restrict__ parent, unsigned long long* __restrict__ key,
-                            int* __restrict__ cell) {
+                            const int* __restrict__ parent, const int* __restrict__ hint,
+                            unsigned long long* __restrict__ key, int* __restrict__ cell) {
   const int c = b + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= e) return;
   const bool has = off[c + 1] > off[c];
   const unsigned cls = (unsigned)(octant[c] * 8 + octant[parent[c]]);
-  key[c - b] = has ? (((unsigned long long)cls << 32) | (unsigned)c) : ~0ULL;
+  const unsigned h = hint ? (unsigned)min(max(hint[c], 0), 15) : 0u;
+  key[c - b] = has ? (((unsigned long long)cls << 36) | ((unsigned long long)h << 32) | (unsigned)c) : ~0ULL;
   cell[c - b] = c;
 }
 
@@ -174,6 +178,8 @@ int read_int(const int* d, hipStream_t st) {
 
 void build_m2l_schedule(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
+  S.tile_pairs_ready = false;  // statistics and maps of the previous tiles
+  S.cell_tile_ready = false;
   const int nc = t.ncells;
   const long long m = t.n_m2l;
   // tiles, one level at a time
@@ -208,8 +214,8 @@ void build_m2l_schedule(Tree& t, hipStream_t st) {
       k_has_m2l<<<blocks(k, 256), 256, 0, st>>>(b, e, t.m2l_off.get(), fl.get());
       scan_ex(t.scratch, fl.get(), rk.get(), k, st);
       const int ncnt = read_int(rk.get() + k - 1, st) + read_int(fl.get() + k - 1, st);
-      k_tile_keys<<<blocks(k, 256), 256, 0, st>>>(b, e, t.m2l_off.get(), t.octant.get(), t.parent.get(), k0.get(),
-                                                  c0.get());
+      k_tile_keys<<<blocks(k, 256), 256, 0, st>>>(b, e, t.m2l_off.get(), t.octant.get(), t.parent.get(),
+                                                  t.tile_hint_ptr(), k0.get(), c0.get());
       size_t bytes = 0;
       FMM_HIP(rocprim::radix_sort_pairs(nullptr, bytes, k0.get(), k1.get(), c0.get(), c1.get(), (size_t)k, 0, 64, st));
       void* tmp = t.scratch.get(bytes);

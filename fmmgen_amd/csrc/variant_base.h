@@ -430,6 +430,21 @@ void field_scale(Tree& t, hipStream_t st) {
   t.judge_scale = keep > 0 ? sqrt(sum / keep) : 0.0;
 }
 
+// The learned cell orders for the tree's current sources (t.cell_q),
+// without an evaluation: the order hints of the M2L tiling.
+void predict_cell_orders(Tree& t, hipStream_t st) {
+  const int n = (int)t.n, nl = t.nleaves;
+  t.pS.alloc((size_t)std::max(n, 1) * SS);
+  k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+  field_scale(t, st);
+  leaf_features(t, SS, st);
+  t.leaf_q.alloc(std::max(nl, 1));
+  t.leaf_y.alloc((size_t)std::max(nl, 1) * kMlpO);
+  mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), t.leaf_y.get(), nullptr, 0.f, 0.f, 0, 0, st);
+  allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, std::max(PMIN, kMlpOrder0), t.order, st);
+  orders_from_leaves(t, st);
+}
+
 void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   FMM_CHECK(t.has_model, kInvalidArgument, "no order model set on the tree");
   const int n = (int)t.n, nl = t.nleaves;
@@ -585,6 +600,7 @@ const Variant FMM_VARIANT_NAME = {FMM_VARIANT_NS::SO,
                                   &FMM_VARIANT_NS::evaluate,
                                   &FMM_VARIANT_NS::evaluate_learned,
                                   &FMM_VARIANT_NS::field_scale,
+                                  &FMM_VARIANT_NS::predict_cell_orders,
                                   &FMM_VARIANT_NS::op_batch,
                                   &FMM_VARIANT_NS::direct};
 #endif

@@ -189,7 +189,10 @@ int fmm_tree_leaf_error(fmm_tree_t tree, const double* d_F, const double* d_Fref
 /* Leaf cell ids in canonical leaf order: h_leaves [nleaves]. */
 int fmm_tree_leaves(fmm_tree_t tree, long long* h_leaves);
 /* Network (copied, converted to FP16), tolerance and safety margin (in
- * log10 units) of the learned modes. */
+ * log10 units) of the learned modes.  For charges it also re-forms the M2L
+ * tiles with the cell orders the network predicts for the tree's current
+ * sources as a key inside each tile class (a tile runs at the largest order
+ * of its cells); synchronises the default stream. */
 int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, double margin);
 /* Statistics of the last learned evaluation, h_stats [10]: largest order
  * used, judge targets above tol * rms |F| (a rerun happens when the
