@@ -62,39 +62,63 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
                           out + (size_t)i * out_stride);
 }
 
-// Sampled judge of the learned order: FP64 direct sum (self excluded) at
-// sorted slots; block (t, b) sums the sources of slice b of target t and
-// adds its partial sum atomically into the zeroed Fd.
+// Sampled judge of the learned order: FP64 direct sums (self excluded) at
+// sorted slots.  A block takes kJudgeTile targets (held in registers, so
+// each source read serves kJudgeTile pairs) against one slice of
+// kJudgeSlice sources, and adds its partial sums atomically into the
+// zero-filled Fd[J][OUT].
 constexpr int kJudgeSlice = 16384;
-
-__global__ void k_judge_direct(int n, const double* __restrict__ px, const double* __restrict__ py,
-                               const double* __restrict__ pz, const double* __restrict__ pS,
-                               const int* __restrict__ slots, int J, double* __restrict__ Fd) {
-  const int t = blockIdx.x;
-  if (t >= J) return;
-  const int i = slots[t];
-  const double xi = px[i], yi = py[i], zi = pz[i];
-  double acc[OUT];
+constexpr int kJudgeTile = 8;
+__global__ void __launch_bounds__(256) k_judge_direct(int n, const double* __restrict__ px,
+                                                      const double* __restrict__ py, const double* __restrict__ pz,
+                                                      const double* __restrict__ pS, const int* __restrict__ slots,
+                                                      int J, double* __restrict__ Fd) {
+  const int t0 = blockIdx.x * kJudgeTile;
+  int ti[kJudgeTile];
+  double tx[kJudgeTile], ty[kJudgeTile], tz[kJudgeTile], acc[kJudgeTile][OUT];
 #pragma unroll
-  for (int k = 0; k < OUT; ++k) acc[k] = 0.0;
+  for (int u = 0; u < kJudgeTile; ++u) {
+    const int tt = min(t0 + u, J - 1);  // a short last tile repeats its last target (not stored)
+    ti[u] = slots[tt];
+    tx[u] = px[ti[u]];
+    ty[u] = py[ti[u]];
+    tz[u] = pz[ti[u]];
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) acc[u][k] = 0.0;
+  }
   const int b = blockIdx.y * kJudgeSlice, e = min(n, b + kJudgeSlice);
-  for (int j = b + threadIdx.x; j < e; j += blockDim.x) {
-    if (j == i) continue;
-    fmmgen_ops::P2P<double>(xi - px[j], yi - py[j], zi - pz[j], pS + (size_t)j * SS, acc);
-  }
-  __shared__ double red[256][OUT];
+  for (int j = b + (int)threadIdx.x; j < e; j += blockDim.x) {
+    const double xj = px[j], yj = py[j], zj = pz[j];
+    double sj[SS];
 #pragma unroll
-  for (int k = 0; k < OUT; ++k) red[threadIdx.x][k] = acc[k];
+    for (int k = 0; k < SS; ++k) sj[k] = pS[(size_t)j * SS + k];
+#pragma unroll
+    for (int u = 0; u < kJudgeTile; ++u) {
+      double dx = tx[u] - xj, dy = ty[u] - yj, dz = tz[u] - zj;
+      double s[SS];
+      const bool self = j == ti[u];
+#pragma unroll
+      for (int k = 0; k < SS; ++k) s[k] = self ? 0.0 : sj[k];
+      if (self) dx = 1.0;  // finite distance, zero strength
+      fmmgen_ops::P2P<double>(dx, dy, dz, s, acc[u]);
+    }
+  }
+  // wave sums, block sums in LDS, one atomic per block and value
+  __shared__ double red[4][kJudgeTile * OUT];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int u = 0; u < kJudgeTile; ++u)
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) {
+      double v = acc[u][k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) red[wave][u * OUT + k] = v;
+    }
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-#pragma unroll
-      for (int k = 0; k < OUT; ++k) red[threadIdx.x][k] += red[threadIdx.x + s][k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int k = 0; k < OUT; ++k) atomicAdd(&Fd[(size_t)t * OUT + k], red[0][k]);
+  const int i = threadIdx.x;
+  if (i < kJudgeTile * OUT && t0 + i / OUT < J)
+    atomicAdd(&Fd[(size_t)t0 * OUT + i], red[0][i] + red[1][i] + red[2][i] + red[3][i]);
 }
 
 // Judge check (one block): a target fails when |F - Fd| over the field
@@ -156,35 +180,80 @@ __global__ void k_fill_int(long long n, int* __restrict__ a) {
 // and the merged particle ranges of its other P2P leaves; they are staged
 // 64 at a time through LDS with coalesced loads.  A leaf whose sources span
 // several items (a small leaf beside a dense cluster) adds its partial sums
-// atomically into a zeroed output; other leaves store.  A leaf of nt <= 32
-// particles is replicated over 64 / pow2(nt) lane slices that take
-// alternate sources of each chunk, and the slices are summed at the end.
+// atomically into a zeroed output; other leaves store.  Targets go 64 to a
+// round; a round of r targets runs as two power-of-two pieces (r = 40 as
+// 32 + 8, r = 20 as 16 + 4), each piece's targets replicated over 64 / p
+// lane slices that take alternate sources of every staged chunk, so fewer
+// lanes idle than with one pow2(r) piece (40 of 64 lanes busy for r = 40).
 //   T = double: FP64 throughout (the reference arithmetic).
 //   T = float : dx = float(x_t - x_s) from FP64 positions (no cancellation
 //               loss for close pairs), the generated P2P body in FP32, and
-//               compensated (Kahan) accumulation of each 64-source chunk's
-//               partial sum into the running total.
+//               compensated (Kahan) accumulation of each chunk's partial
+//               sum into the running total.
 // The result is written in FP64 per sorted slot.
 template <typename T, bool SELF>
-__device__ __forceinline__ void p2p_chunk(int filled, int sl, int nslice, int tslot, double xt, double yt,
-                                          double zt, const double* sx, const double* sy, const double* sz,
-                                          const T* sS, const int* sid, T (&part)[OUT]) {
-  for (int j = sl; j < filled; j += nslice) {
-    T dx, dy, dz;
-    dx = (T)(xt - sx[j]);
-    dy = (T)(yt - sy[j]);
-    dz = (T)(zt - sz[j]);
-    T sj[SS];
+__device__ __forceinline__ void p2p_pair(int j, int tslot, double xt, double yt, double zt, const double* sx,
+                                         const double* sy, const double* sz, const T* sS, const int* sid,
+                                         T (&part)[OUT]) {
+  T dx = (T)(xt - sx[j]), dy = (T)(yt - sy[j]), dz = (T)(zt - sz[j]);
+  T sj[SS];
 #pragma unroll
-    for (int k = 0; k < SS; ++k) sj[k] = sS[j * SS + k];
-    if (SELF && sid[j] == tslot) {  // self pair: zero source at unit distance
-      dx = T(1);
-      dy = T(0);
-      dz = T(0);
+  for (int k = 0; k < SS; ++k) sj[k] = sS[j * SS + k];
+  if (SELF && sid[j] == tslot) {  // self pair: zero source at unit distance
+    dx = T(1);
+    dy = T(0);
+    dz = T(0);
 #pragma unroll
-      for (int k = 0; k < SS; ++k) sj[k] = T(0);
+    for (int k = 0; k < SS; ++k) sj[k] = T(0);
+  }
+  fmmgen_ops::P2P<T>(dx, dy, dz, sj, part);
+}
+
+// Sources sl, sl + NS, sl + 2 NS, ... of a chunk of `filled`: a uniform trip
+// count (filled / NS) unrolled with immediate LDS offsets, then one masked
+// remainder pair.
+template <typename T, bool SELF, int NS>
+__device__ __forceinline__ void p2p_chunk(int filled, int sl, int tslot, double xt, double yt, double zt,
+                                          const double* sx, const double* sy, const double* sz, const T* sS,
+                                          const int* sid, T (&part)[OUT]) {
+  const int full = filled / NS;
+  const double *x = sx + sl, *y = sy + sl, *z = sz + sl;
+  const T* S = sS + sl * SS;
+  const int* id = sid + sl;
+#pragma unroll 4
+  for (int k = 0; k < full; ++k)
+    p2p_pair<T, SELF>(k * NS, tslot, xt, yt, zt, x, y, z, S, id, part);
+  if (sl < filled - full * NS) p2p_pair<T, SELF>(full * NS, tslot, xt, yt, zt, x, y, z, S, id, part);
+}
+
+template <typename T, bool SELF>
+__device__ __forceinline__ void p2p_chunk_any(int nslice, int filled, int sl, int tslot, double xt, double yt,
+                                              double zt, const double* sx, const double* sy, const double* sz,
+                                              const T* sS, const int* sid, T (&part)[OUT]) {
+  switch (nslice) {
+#define FMM_P2P_NS(ns) \
+  case ns:             \
+    p2p_chunk<T, SELF, ns>(filled, sl, tslot, xt, yt, zt, sx, sy, sz, sS, sid, part); \
+    break;
+    FMM_P2P_NS(1) FMM_P2P_NS(2) FMM_P2P_NS(4) FMM_P2P_NS(8) FMM_P2P_NS(16) FMM_P2P_NS(32) FMM_P2P_NS(64)
+#undef FMM_P2P_NS
+  }
+}
+
+// Chunk partial sum into a running total: compensated (Kahan) in FP32,
+// plain in FP64.
+template <typename T>
+__device__ __forceinline__ void p2p_accumulate(const T (&part)[OUT], T (&sum)[OUT], T (&comp)[OUT]) {
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) {
+    if constexpr (sizeof(T) == 4) {
+      const T y = part[k] - comp[k];
+      const T s2 = sum[k] + y;
+      comp[k] = (s2 - sum[k]) - y;
+      sum[k] = s2;
+    } else {
+      sum[k] += part[k];
     }
-    fmmgen_ops::P2P<T>(dx, dy, dz, sj, part);
   }
 }
 
@@ -207,39 +276,45 @@ __global__ __launch_bounds__(64) void k_p2p_leaf(int nitems, int nblocks, const 
   const int A = it_leaf[it];
   const int lane = threadIdx.x;
   const int t0 = pstart[A], nt = pcount[A];
-  int tpad = 1;
-  while (tpad < nt && tpad < 64) tpad <<= 1;
-  const int nslice = 64 / tpad;
   const int rb = rng_off[A], re = rng_off[A + 1];
   // a leaf cut into several items accumulates atomically
   // (the items of a leaf are contiguous)
   const bool atomic = split && ((it > 0 && it_leaf[it - 1] == A) || (it + 1 < nitems && it_leaf[it + 1] == A));
   for (int tb = 0; tb < nt; tb += 64) {
-    const int tl = tb + (lane % tpad), sl = lane / tpad;
-    const bool active = tl < nt;
-    const int tslot = t0 + tl;
-    const double xt = active ? px[tslot] : 0.0, yt = active ? py[tslot] : 0.0, zt = active ? pz[tslot] : 0.0;
-    T sum[OUT], comp[OUT];
+    // this round's r targets as two power-of-two pieces: pa = the largest
+    // power of two <= r (64 at most) and pb = r - pa rounded up to a power
+    // of two (0 when r is one); piece targets are replicated over 64 / p
+    // lane slices taking alternate sources of each chunk
+    const int r = min(64, nt - tb);
+    const int pa = 1 << (31 - __clz(r));
+    const int pb = r > pa ? 1 << (32 - __clz(r - pa - 1)) : 0;
+    const int sla = lane / pa, tia = lane % pa;
+    const int slb = pb ? lane / pb : 0, tib = pb ? pa + lane % pb : r;
+    const bool actb = tib < r;
+    const int ta = t0 + tb + tia, tbs = t0 + tb + (actb ? tib : tia);
+    const double xa = px[ta], ya = py[ta], za = pz[ta];
+    const double xb = px[tbs], yb = py[tbs], zb = pz[tbs];
+    T suma[OUT], compa[OUT], sumb[OUT], compb[OUT];
 #pragma unroll
-    for (int k = 0; k < OUT; ++k) sum[k] = comp[k] = T(0);
+    for (int k = 0; k < OUT; ++k) suma[k] = compa[k] = sumb[k] = compb[k] = T(0);
     // source sequence: the own leaf (virtual range rb - 1, self pairs
     // masked), then the merged ranges rb .. re - 1; this item covers
     // it_n[it] sources from range it_r[it], offset it_o[it]
-    int r = it_r[it], o = it_o[it], left = it_n[it];
+    int rg = it_r[it], o = it_o[it], left = it_n[it];
     while (left > 0) {
-      const bool has_self = r < rb;
+      const bool has_self = rg < rb;
       int filled = 0;
       __syncthreads();  // previous chunk consumed
-      while (filled < 64 && left > 0 && r < re) {
-        const int rs = (r < rb) ? t0 : rng_start[r];
-        const int rl = (r < rb) ? nt : rng_len[r];
+      while (filled < 64 && left > 0 && rg < re) {
+        const int rs = (rg < rb) ? t0 : rng_start[rg];
+        const int rl = (rg < rb) ? nt : rng_len[rg];
         const int take = min(min(64 - filled, rl - o), left);
         if (lane >= filled && lane < filled + take) {
           const int j = rs + o + (lane - filled);
           sx[lane] = px[j];
           sy[lane] = py[j];
           sz[lane] = pz[j];
-          sid[lane] = (r < rb) ? j : -1;
+          sid[lane] = (rg < rb) ? j : -1;
 #pragma unroll
           for (int k = 0; k < SS; ++k) sS[lane * SS + k] = (T)pS[(size_t)j * SS + k];
         }
@@ -247,7 +322,7 @@ __global__ __launch_bounds__(64) void k_p2p_leaf(int nitems, int nblocks, const 
         o += take;
         left -= take;
         if (o == rl) {
-          ++r;
+          ++rg;
           o = 0;
         }
       }
@@ -256,32 +331,39 @@ __global__ __launch_bounds__(64) void k_p2p_leaf(int nitems, int nblocks, const 
       T part[OUT];
 #pragma unroll
       for (int k = 0; k < OUT; ++k) part[k] = T(0);
-      if (active) {
-        if (has_self)
-          p2p_chunk<T, true>(filled, sl, nslice, tslot, xt, yt, zt, sx, sy, sz, sS, sid, part);
-        else
-          p2p_chunk<T, false>(filled, sl, nslice, tslot, xt, yt, zt, sx, sy, sz, sS, sid, part);
-      }
+      if (has_self)
+        p2p_chunk_any<T, true>(64 / pa, filled, sla, ta, xa, ya, za, sx, sy, sz, sS, sid, part);
+      else
+        p2p_chunk_any<T, false>(64 / pa, filled, sla, ta, xa, ya, za, sx, sy, sz, sS, sid, part);
+      p2p_accumulate<T>(part, suma, compa);
+      if (pb) {
 #pragma unroll
-      for (int k = 0; k < OUT; ++k) {
-        if constexpr (kKahan) {
-          const T y = part[k] - comp[k];
-          const T s2 = sum[k] + y;
-          comp[k] = (s2 - sum[k]) - y;
-          sum[k] = s2;
-        } else {
-          sum[k] += part[k];
+        for (int k = 0; k < OUT; ++k) part[k] = T(0);
+        if (actb) {
+          if (has_self)
+            p2p_chunk_any<T, true>(64 / pb, filled, slb, tbs, xb, yb, zb, sx, sy, sz, sS, sid, part);
+          else
+            p2p_chunk_any<T, false>(64 / pb, filled, slb, tbs, xb, yb, zb, sx, sy, sz, sS, sid, part);
         }
+        p2p_accumulate<T>(part, sumb, compb);
       }
     }
-    // fold the slices of each target into slice 0
+    // fold each piece's slices; slice 0 of lane i holds target i of the piece
 #pragma unroll
     for (int k = 0; k < OUT; ++k) {
-      double v = kKahan ? (double)sum[k] - (double)comp[k] : (double)sum[k];
-      for (int w = tpad; w < 64; w <<= 1) v += __shfl_xor(v, w, 64);
-      if (active && sl == 0) {
-        if (atomic) atomicAdd(&F[(size_t)tslot * OUT + k], v);
-        else F[(size_t)tslot * OUT + k] = v;
+      double va = kKahan ? (double)suma[k] - (double)compa[k] : (double)suma[k];
+      for (int w = pa; w < 64; w <<= 1) va += __shfl_xor(va, w, 64);
+      if (sla == 0) {
+        if (atomic) atomicAdd(&F[(size_t)ta * OUT + k], va);
+        else F[(size_t)ta * OUT + k] = va;
+      }
+      if (pb) {
+        double vb = kKahan ? (double)sumb[k] - (double)compb[k] : (double)sumb[k];
+        for (int w = pb; w < 64; w <<= 1) vb += __shfl_xor(vb, w, 64);
+        if (slb == 0 && actb) {
+          if (atomic) atomicAdd(&F[(size_t)tbs * OUT + k], vb);
+          else F[(size_t)tbs * OUT + k] = vb;
+        }
       }
     }
   }
@@ -411,7 +493,7 @@ void field_scale(Tree& t, hipStream_t st) {
   t.judge_scale = 0.0;
   if (J == 0) return;
   k_fill_zero<<<nblk((long long)J * OUT, 256), 256, 0, st>>>((long long)J * OUT, t.judge_Fd.get());
-  const dim3 g(J, (n + kJudgeSlice - 1) / kJudgeSlice);
+  const dim3 g((J + kJudgeTile - 1) / kJudgeTile, (n + kJudgeSlice - 1) / kJudgeSlice);
   k_judge_direct<<<g, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
                                     t.judge_Fd.get());
   std::vector<double> h((size_t)J * OUT);
