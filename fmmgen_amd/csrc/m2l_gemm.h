@@ -29,6 +29,7 @@
 // power of two (2^(level_A - level_B)) applied exactly with ldexp.
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 #include "fmm_common.h"
@@ -169,33 +170,120 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * per + k;
 }
 
-// Steps of wave wv of a W-wave block (row blocks I = wv mod W), ordered by
-// the lowest total degree deg(row block) + deg(slice) (cnt[q]: the steps of
-// lowest degree <= q).
+// Steps of wave wv of a W-wave block, balanced so that every wave issues
+// about NSTEP / W MFMAs.  Low-degree row blocks are valid for nearly every
+// slice and high ones for few, so whole rows dealt round robin would leave
+// wave 0 with about half of the work.  Rows I >= R0 are dealt round robin
+// (I = wv mod W); the steps of the rows I < R0 (R0 the least for which no
+// wave's round-robin share exceeds NSTEP / W) are taken in slice-major order
+// and cut into consecutive runs that fill every wave up to its share.  A
+// wave holds one accumulator per row it touches (rows[0 .. nrow-1]; a low
+// row cut between waves is added into L by each).  Within the wave steps
+// are ordered by slice K, so consecutive steps of one slice share their B
+// operand read.
 template <int P, int SO, int W, int WV>
 struct WavePlan {
   using Plan = M2LPlan<P, SO>;
   struct Tab {
     short s[Plan::NSTEP + 1];
-    int n;
-    int cnt[P + 1];
+    int n, nrow;
+    short rows[Plan::NI + 1];
+    short slot[Plan::NI + 1];  // accumulator of row I (-1: none)
   };
+  static constexpr int rowcount(int I) {
+    int c = 0;
+    for (int K = 0; K < Plan::NK; ++K)
+      if (I < Plan::tab.nI[K]) ++c;
+    return c;
+  }
+  static constexpr int high_share(int R0, int w) {
+    int c = 0;
+    for (int I = R0; I < Plan::NI; ++I)
+      if (I % W == w) c += rowcount(I);
+    return c;
+  }
+  static constexpr Tab make() {
+    Tab t{};
+    const int N = Plan::NSTEP;
+    int R0 = 0;
+    for (; R0 < Plan::NI; ++R0) {
+      int mx = 0;
+      for (int w = 0; w < W; ++w) mx = high_share(R0, w) > mx ? high_share(R0, w) : mx;
+      if ((long long)mx * W <= N) break;
+    }
+    int lowN = 0;
+    for (int K = 0; K < Plan::NK; ++K) lowN += Plan::tab.nI[K] < R0 ? Plan::tab.nI[K] : R0;
+    // low-step run of wave w: [cut(w), cut(w + 1))
+    auto cut = [&](int w) {
+      if (w >= W) return lowN;
+      long long h = 0;
+      for (int v = 0; v < w; ++v) h += high_share(R0, v);
+      long long c = (long long)N * w / W - h;
+      return (int)(c < 0 ? 0 : (c > lowN ? lowN : c));
+    };
+    const int lb = cut(WV), le = cut(WV + 1) > lb ? cut(WV + 1) : lb;
+    bool used[Plan::NSTEP + 1] = {};
+    int idx = 0;
+    for (int K = 0; K < Plan::NK; ++K)
+      for (int I = 0; I < Plan::tab.nI[K] && I < R0; ++I, ++idx)
+        if (idx >= lb && idx < le) used[Plan::tab.base[K] + I] = true;
+    for (int K = 0; K < Plan::NK; ++K)
+      for (int I = R0; I < Plan::tab.nI[K]; ++I)
+        if (I % W == WV) used[Plan::tab.base[K] + I] = true;
+    int n = 0;
+    for (int st = 0; st < N; ++st)
+      if (used[st]) t.s[n++] = (short)st;  // K-major table: slice, then row order
+    t.n = n;
+    for (int I = 0; I <= Plan::NI; ++I) t.slot[I] = -1;
+    int nr = 0;
+    for (int I = 0; I < Plan::NI; ++I) {
+      bool has = false;
+      for (int i = 0; i < n; ++i) has = has || Plan::tab.sI[t.s[i]] == I;
+      if (has) {
+        t.slot[I] = (short)nr;
+        t.rows[nr++] = (short)I;
+      }
+    }
+    t.nrow = nr;
+    return t;
+  }
+  static constexpr Tab tab = make();
+};
+
+// Round-robin plan: wave wv takes the row blocks I = wv mod W, its steps
+// ordered by the lowest total degree deg(row block) + deg(slice).  At
+// orders below 10 the block time is dominated by staging and the L adds
+// rather than by the MFMAs, and this plan (fewer rows per wave, so fewer
+// adds) measures faster than the balanced one despite its uneven waves.
+template <int P, int SO, int W, int WV>
+struct WavePlanRR {
+  using Plan = M2LPlan<P, SO>;
+  using Tab = typename WavePlan<P, SO, W, WV>::Tab;
   static constexpr int lowdeg(int s) {
     return slot_degree(16 * Plan::tab.sI[s]) + slot_degree(4 * Plan::tab.sK[s] + Plan::OFF);
   }
   static constexpr Tab make() {
     Tab t{};
     int n = 0;
-    for (int d = 0; d <= P; ++d) {
-      for (int s = 0; s < Plan::NSTEP; ++s)
-        if (Plan::tab.sI[s] % W == WV && lowdeg(s) == d) t.s[n++] = (short)s;
-      t.cnt[d] = n;
-    }
+    for (int d = 0; d <= P; ++d)
+      for (int st = 0; st < Plan::NSTEP; ++st)
+        if (Plan::tab.sI[st] % W == WV && lowdeg(st) == d) t.s[n++] = (short)st;
     t.n = n;
+    int nr = 0;
+    for (int I = 0; I <= Plan::NI; ++I) t.slot[I] = -1;
+    for (int I = WV; I < Plan::NI; I += W) {
+      t.slot[I] = (short)nr;
+      t.rows[nr++] = (short)I;
+    }
+    t.nrow = n > 0 ? nr : 0;
     return t;
   }
   static constexpr Tab tab = make();
 };
+
+// The plan a kernel of order P uses (measured choice).
+template <int P, int SO, int W, int WV>
+using WavePlanFor = std::conditional_t<(P >= 10), WavePlan<P, SO, W, WV>, WavePlanRR<P, SO, W, WV>>;
 
 // Waves per block: the LDS tables of a chunk are shared by W waves.
 template <int P, int SO, typename T>
@@ -256,8 +344,8 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 
   dispatch_wave<W>(w, [&](auto WV) {
     constexpr int wv = WV;
-    using WP = WavePlan<P, SO, W, wv>;
-    constexpr int NIW = (NI - wv + W - 1) / W;  // row blocks wv, wv + W, ...
+    using WP = WavePlanFor<P, SO, W, wv>;
+    constexpr int NIW = WP::tab.nrow;  // accumulators: one per row block touched
     constexpr int NSW = WP::tab.n;
     V4 acc[NIW > 0 ? NIW : 1];
 #pragma unroll
@@ -327,7 +415,9 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
               // an empty entry (offset 255) reads a zero slot when the
               // step's base is within 255 of MS, else it is masked here
               if constexpr (MS - base > 255) av[u] = off == 255 ? T(0) : av[u];
-              bv[u] = sB[col * MSP + 4 * K + kk];
+              constexpr int KP = (u > 0) ? Plan::tab.sK[WP::tab.s[i > 0 ? i - 1 : 0]] : -1;
+              if constexpr (K == KP) bv[u] = bv[u > 0 ? u - 1 : 0];
+              else bv[u] = sB[col * MSP + 4 * K + kk];
             }
           });
           static_for<kBatch>([&](auto u) {
@@ -335,7 +425,8 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
             if constexpr (i < NSW) {
               constexpr int s = WP::tab.s[i];
               constexpr int I = Plan::tab.sI[s];
-              acc[I / W] = mfma16x16x4(av[u], bv[u], acc[I / W]);
+              constexpr int A = WP::tab.slot[I];
+              acc[A] = mfma16x16x4(av[u], bv[u], acc[A]);
             }
           });
         }
@@ -345,7 +436,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
     if (cell < 0) return;
     T* Lc = L + (size_t)cell * lstride;
     static_for<NIW>([&](auto ii) {
-      constexpr int I = wv + W * ii;
+      constexpr int I = WP::tab.rows[ii];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = 16 * I + acc_row<T>(kk, r);
