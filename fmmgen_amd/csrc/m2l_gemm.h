@@ -251,23 +251,19 @@ struct WavePlan {
 };
 
 // Round-robin plan: wave wv takes the row blocks I = wv mod W, its steps
-// ordered by the lowest total degree deg(row block) + deg(slice).  At
-// orders below 10 the block time is dominated by staging and the L adds
+// in slice-major order (consecutive steps of one slice share the B read).
+// At orders below 10 the block time is dominated by staging and the L adds
 // rather than by the MFMAs, and this plan (fewer rows per wave, so fewer
 // adds) measures faster than the balanced one despite its uneven waves.
 template <int P, int SO, int W, int WV>
 struct WavePlanRR {
   using Plan = M2LPlan<P, SO>;
   using Tab = typename WavePlan<P, SO, W, WV>::Tab;
-  static constexpr int lowdeg(int s) {
-    return slot_degree(16 * Plan::tab.sI[s]) + slot_degree(4 * Plan::tab.sK[s] + Plan::OFF);
-  }
   static constexpr Tab make() {
     Tab t{};
     int n = 0;
-    for (int d = 0; d <= P; ++d)
-      for (int st = 0; st < Plan::NSTEP; ++st)
-        if (Plan::tab.sI[st] % W == WV && lowdeg(st) == d) t.s[n++] = (short)st;
+    for (int st = 0; st < Plan::NSTEP; ++st)
+      if (Plan::tab.sI[st] % W == WV) t.s[n++] = (short)st;
     t.n = n;
     int nr = 0;
     for (int I = 0; I <= Plan::NI; ++I) t.slot[I] = -1;
