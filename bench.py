@@ -114,7 +114,7 @@ def field_error(F, Fref, tg, so):
     return ferr, perr
 
 
-def roofline_of(phases, info, order, so, mode, m2l_macs=None):
+def roofline_of(phases, info, order, so, mode, m2l_macs=None, config="c4"):
     """Roofline entry of the dominant kernel (P2P: VALU, M2L: MFMA); times are
     hipEvent phase times on the evaluation stream."""
     dom = max(("P2P", "M2L"), key=lambda k: phases[k])
@@ -129,7 +129,22 @@ def roofline_of(phases, info, order, so, mode, m2l_macs=None):
         bound = "mfma"
     achieved = flops / (phases[dom] * 1e-3) / 1e12
     return {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": None, "avg_ms": phases[dom], "flop_per_launch": flops}
+            "frac": achieved / peak, "traffic": measured_traffic(config, mode, dom), "avg_ms": phases[dom],
+            "flop_per_launch": flops}
+
+
+def measured_traffic(config, mode, kernel):
+    """HBM bytes per evaluation of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement of the same workload
+    (profiles/*_traffic.json, scaled by a copy kernel of known size), or
+    None where no such measurement exists."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{kernel.lower()}_{mode}_{config}_blobs_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d["hbm_bytes_per_eval"] if d.get("workload") == config and d.get("mode") == mode else None
 
 
 def main():
@@ -213,7 +228,7 @@ def main():
     if stats:
         extra["learned"] = stats
         print(f"# learned stats {stats}", file=sys.stderr, flush=True)
-    roofline = roofline_of(avg, info, cfg["order"], so, mode, stats["m2l_macs"] if stats else None)
+    roofline = roofline_of(avg, info, cfg["order"], so, mode, stats["m2l_macs"] if stats else None, args.config)
 
     base = None
     if not args.no_cpu_baseline:
