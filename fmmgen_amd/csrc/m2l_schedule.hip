@@ -284,16 +284,20 @@ void build_m2l_schedule(Tree& t, hipStream_t st) {
                                                 S.tile_group_off.get());
   k_close_offsets<<<1, 1, 0, st>>>(S.ntiles, S.tile_group_off.get(), S.ngroups);
   FMM_HIP(hipStreamSynchronize(st));
-  // work items: runs of at most kGroupsPerChunk groups of one tile
+  // work items: runs of groups of one tile
   std::vector<int> goff(S.ntiles + 1);
   FMM_HIP(hipMemcpy(goff.data(), S.tile_group_off.get(), sizeof(int) * (S.ntiles + 1), hipMemcpyDeviceToHost));
   std::vector<int> ct, c0, c1;
-  for (int tl = 0; tl < S.ntiles; ++tl)
-    for (int g = goff[tl]; g < goff[tl + 1]; g += kGroupsPerChunk) {
+  // a tile's groups cut into equal runs of at most kGroupsPerChunk
+  for (int tl = 0; tl < S.ntiles; ++tl) {
+    const int G = goff[tl + 1] - goff[tl];
+    const int nch = (G + kGroupsPerChunk - 1) / kGroupsPerChunk;
+    for (int k = 0; k < nch; ++k) {
       ct.push_back(tl);
-      c0.push_back(g);
-      c1.push_back(std::min(goff[tl + 1], g + kGroupsPerChunk));
+      c0.push_back(goff[tl] + (int)((long long)G * k / nch));
+      c1.push_back(goff[tl] + (int)((long long)G * (k + 1) / nch));
     }
+  }
   S.nchunks = (int)ct.size();
   S.chunk_tile.alloc(S.nchunks);
   S.chunk_g0.alloc(S.nchunks);
