@@ -87,6 +87,81 @@ __global__ __launch_bounds__(64) void k_p2m_mixed(int nleaves, const int* __rest
   });
 }
 
+// FP32 P2M of charges, one wave per leaf: the normalised multipole of a
+// charge q at u = (x_centre - x_particle) / r is
+// M[a,b,c] = q u^a v^b w^c / (a! b! c!) (the closed form of the generated
+// P2M_S body; multi-index layout of the expansion).  Lanes own the slots lane, lane + 64, ...;
+// 64 particles at a time stage their scaled powers u^k / k! (k <= P) in
+// LDS, and every lane adds q X[a] Y[b] Z[c] for its slots.  Particle data
+// are read coalesced once per leaf, and the lanes stay converged (the
+// thread-per-(leaf, chunk) layout loops each lane over its own leaf).
+template <int P>
+__global__ __launch_bounds__(64) void k_p2m_wave(int nleaves, const int* __restrict__ leaves,
+                                                 const int* __restrict__ pstart, const int* __restrict__ pcount,
+                                                 const double* __restrict__ cx, const double* __restrict__ cy,
+                                                 const double* __restrict__ cz, const double* __restrict__ cr,
+                                                 const double* __restrict__ px, const double* __restrict__ py,
+                                                 const double* __restrict__ pz, const double* __restrict__ pS,
+                                                 float* __restrict__ Mf, double* __restrict__ Mtop, int ntop) {
+  static_assert(SO == 0 || P < 0, "charges only");
+  constexpr int MS = Op<P>::MS, NPL = (MS + 63) / 64, PW = 3 * (P + 1);
+  __shared__ float pw[64 * PW];  // particle k: X[0..P], Y[0..P], Z[0..P]
+  __shared__ float sq[64];
+  const int li = blockIdx.x;
+  if (li >= nleaves) return;
+  const int lane = threadIdx.x;
+  const int c = leaves[li];
+  const double x0 = cx[c], y0 = cy[c], z0 = cz[c], ri = 1.0 / cr[c];
+  int ea[NPL], eb[NPL], ec[NPL];
+  float acc[NPL];
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    const int sl = lane + 64 * i;
+    const MultiIdx m = slot_multi(sl < MS ? sl : 0);
+    ea[i] = m.x;
+    eb[i] = P + 1 + m.y;
+    ec[i] = 2 * (P + 1) + m.z;
+    acc[i] = 0.f;
+  }
+  const int b = pstart[c], e = b + pcount[c];
+  for (int base = b; base < e; base += 64) {
+    const int j = base + lane;
+    __syncthreads();  // previous particles consumed
+    {
+      const bool in = j < e;
+      const float u[3] = {in ? (float)((x0 - px[j]) * ri) : 0.f, in ? (float)((y0 - py[j]) * ri) : 0.f,
+                          in ? (float)((z0 - pz[j]) * ri) : 0.f};
+      sq[lane] = in ? (float)pS[j] : 0.f;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        float v = 1.f;
+        pw[lane * PW + d * (P + 1)] = 1.f;
+#pragma unroll
+        for (int k = 1; k <= P; ++k) {
+          v = v * u[d] * (1.f / (float)k);
+          pw[lane * PW + d * (P + 1) + k] = v;
+        }
+      }
+    }
+    __syncthreads();
+    const int nk = min(64, e - base);
+    for (int k = 0; k < nk; ++k) {
+      const float q = sq[k];
+      const float* w = pw + k * PW;
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) acc[i] = fmaf(q, w[ea[i]] * w[eb[i]] * w[ec[i]], acc[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    const int sl = lane + 64 * i;
+    if (sl < MS) {
+      Mf[(size_t)c * MS + sl] = acc[i];
+      if (c < ntop) Mtop[(size_t)c * MS + sl] = acc[i];
+    }
+  }
+}
+
 // FP32 M2M of the parents of one level (below the FP64 top levels).
 template <int P>
 __global__ __launch_bounds__(64) void k_m2m_mixed(int beg, int end, const int* __restrict__ first_child,
@@ -262,9 +337,14 @@ void far_field_mixed(Tree& t, int mode, int mask, hipStream_t st) {
   FMM_HIP(hipMemsetAsync(t.Lf.get(), 0, sizeof(float) * nc * LS, st));
   if (mask & kPhaseP2M) {
     PhaseTimer pt(t, kTP2M, st);
-    k_p2m_mixed<P><<<dim3(nblk(t.nleaves, 64), (MS + kChunk - 1) / kChunk), 64, 0, st>>>(
-        t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
-        t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.Mf.get(), t.Mtop.get(), ntop);
+    if constexpr (SO == 0)
+      k_p2m_wave<P><<<t.nleaves, 64, 0, st>>>(t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(), t.cx.get(),
+                                              t.cy.get(), t.cz.get(), t.cr.get(), t.px.get(), t.py.get(),
+                                              t.pz.get(), t.pS.get(), t.Mf.get(), t.Mtop.get(), ntop);
+    else
+      k_p2m_mixed<P><<<dim3(nblk(t.nleaves, 64), (MS + kChunk - 1) / kChunk), 64, 0, st>>>(
+          t.nleaves, t.leaves.get(), t.pstart.get(), t.pcount.get(), t.cx.get(), t.cy.get(), t.cz.get(),
+          t.cr.get(), t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.Mf.get(), t.Mtop.get(), ntop);
   }
   if (mask & kPhaseM2M) {
     PhaseTimer pt(t, kTM2M, st);
