@@ -282,6 +282,132 @@ __global__ __launch_bounds__(64) void k_l2l_top(int beg, int end, const int* __r
   });
 }
 
+// FP64 shifts of the top levels for charges, one wave per child (M2M: the
+// children's shifts summed in LDS), lanes over output slots, from the
+// closed forms of the generated bodies (multi-index layout, X[a] = x^a/a!):
+//   M2M: Ms[n] += sum over k <= n (componentwise) of M[n - k] X Y Z[k],
+//        x = (parent - child) / r_parent, M the child's multipole scaled
+//        by 2^-deg (its normalisation at the parent's radius);
+//   L2L: Ls[n] = sum over |n + k| <= P of L[n + k] X Y Z[k],
+//        x = (child - parent) / r_parent, then scaled by 2^-(deg n + 1).
+// Same results as k_m2m_top / k_l2l_top (to FP64 rounding), with the
+// source expansion and the powers in LDS instead of one thread per 32-slot
+// chunk running a long generated body.
+template <int P>
+__global__ __launch_bounds__(512) void k_m2m_top_wave(int beg, int end, const int* __restrict__ first_child,
+                                                      const int* __restrict__ nchild, const double* __restrict__ cx,
+                                                      const double* __restrict__ cy, const double* __restrict__ cz,
+                                                      const double* __restrict__ cr, float* __restrict__ Mf,
+                                                      double* __restrict__ Mtop, int ntop) {
+  static_assert(SO == 0 || P < 0, "charges only");
+  constexpr int MS = Op<P>::MS, NPL = (MS + 63) / 64;
+  __shared__ double sM[8][MS];  // wave w: child w's scaled multipole, then its contribution
+  __shared__ double pw[8][3][P + 1];
+  const int p = beg + blockIdx.x;
+  if (p >= end) return;
+  const int k = nchild[p];
+  if (k == 0) return;  // uniform over the block
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, fc = first_child[p];
+  const double ri = 1.0 / cr[p];
+  constexpr DegTab<512> tab{};
+  double acc[NPL];
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) acc[i] = 0.0;
+  if (w < k) {  // wave w shifts child fc + w
+    const int c = fc + w;
+    for (int j = lane; j < MS; j += 64)
+      sM[w][j] = ldexp(c < ntop ? Mtop[(size_t)c * MS + j] : (double)Mf[(size_t)c * MS + j], -tab.d[j]);
+    if (lane < 3 * (P + 1)) {
+      const int d = lane / (P + 1), a = lane % (P + 1);
+      const double u = d == 0 ? (cx[p] - cx[c]) * ri : (d == 1 ? (cy[p] - cy[c]) * ri : (cz[p] - cz[c]) * ri);
+      double v = 1.0;
+      for (int e = 1; e <= a; ++e) v = v * u / (double)e;
+      pw[w][d][a] = v;
+    }
+  }
+  __syncthreads();
+  if (w < k) {
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const int n = lane + 64 * i;
+      if (n < MS) {
+        const MultiIdx m = slot_multi(n);
+        double sum = 0.0;
+        for (int ia = 0; ia <= m.x; ++ia)
+          for (int jb = 0; jb <= m.y; ++jb) {
+            const double xy = pw[w][0][ia] * pw[w][1][jb];
+            for (int kc = 0; kc <= m.z; ++kc)
+              sum += sM[w][multi_slot(m.x - ia, m.y - jb, m.z - kc)] * (xy * pw[w][2][kc]);
+          }
+        acc[i] = sum;
+      }
+    }
+  }
+  __syncthreads();  // every wave done reading its child
+  if (w < k) {
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const int n = lane + 64 * i;
+      if (n < MS) sM[w][n] = acc[i];
+    }
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < MS; n += 512) {
+    double v = 0.0;
+    for (int c = 0; c < k; ++c) v += sM[c][n];
+    Mtop[(size_t)p * MS + n] = v;
+    Mf[(size_t)p * MS + n] = (float)v;
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_l2l_top_wave(int beg, int end, const int* __restrict__ parent,
+                                                     const double* __restrict__ cx, const double* __restrict__ cy,
+                                                     const double* __restrict__ cz, const double* __restrict__ cr,
+                                                     float* __restrict__ Lf, double* __restrict__ Ltop, int ntop) {
+  static_assert(SO == 0 || P < 0, "charges only");
+  constexpr int LS = Op<P>::LS, NPL = (LS + 63) / 64;
+  __shared__ double sL[LS];
+  __shared__ double pw[3][P + 1];
+  const int c = beg + blockIdx.x;
+  if (c >= end) return;
+  const int lane = threadIdx.x, p = parent[c];
+  const double ri = 1.0 / cr[p];
+  for (int j = lane; j < LS; j += 64) sL[j] = Ltop[(size_t)p * LS + j];
+  if (lane < 3 * (P + 1)) {
+    const int d = lane / (P + 1), a = lane % (P + 1);
+    const double u = d == 0 ? (cx[c] - cx[p]) * ri : (d == 1 ? (cy[c] - cy[p]) * ri : (cz[c] - cz[p]) * ri);
+    double v = 1.0;
+    for (int e = 1; e <= a; ++e) v = v * u / (double)e;
+    pw[d][a] = v;
+  }
+  __syncthreads();
+  constexpr DegTab<512> tab{};
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) {
+    const int n = lane + 64 * i;
+    if (n < LS) {
+      const MultiIdx m = slot_multi(n);
+      const int R = P - (m.x + m.y + m.z);
+      double sum = 0.0;
+      for (int ia = 0; ia <= R; ++ia)
+        for (int jb = 0; jb <= R - ia; ++jb) {
+          const double xy = pw[0][ia] * pw[1][jb];
+          for (int kc = 0; kc <= R - ia - jb; ++kc)
+            sum += sL[multi_slot(m.x + ia, m.y + jb, m.z + kc)] * (xy * pw[2][kc]);
+        }
+      const double v = ldexp(sum, -(tab.d[n] + 1));
+      const size_t o = (size_t)c * LS + n;
+      if (c < ntop) {
+        Ltop[o] = (double)Lf[o] + v;
+        Lf[o] = (float)Ltop[o];
+      } else {
+        Lf[o] += (float)v;
+      }
+    }
+  }
+}
+
 static __global__ void k_seed_root(int LS, const float* __restrict__ Lf, double* __restrict__ Ltop) {
   for (int j = threadIdx.x; j < LS; j += blockDim.x) Ltop[j] = (double)Lf[j];
 }
@@ -351,10 +477,15 @@ void far_field_mixed(Tree& t, int mode, int mask, hipStream_t st) {
     for (int l = t.depth - 2; l >= 0; --l) {
       const int b = t.level_off[l], e = t.level_off[l + 1];
       const dim3 g(nblk(e - b, 64), (MS + kChunk - 1) / kChunk);
-      if (top64 && l < kTopLevels)
-        k_m2m_top<P><<<g, 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(),
-                                       t.cz.get(), t.cr.get(), t.Mf.get(), t.Mtop.get(), ntop);
-      else
+      if (top64 && l < kTopLevels) {
+        if constexpr (SO == 0)
+          k_m2m_top_wave<P><<<e - b, 512, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(),
+                                                  t.cy.get(), t.cz.get(), t.cr.get(), t.Mf.get(), t.Mtop.get(),
+                                                  ntop);
+        else
+          k_m2m_top<P><<<g, 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(),
+                                         t.cz.get(), t.cr.get(), t.Mf.get(), t.Mtop.get(), ntop);
+      } else
         k_m2m_mixed<P><<<g, 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(),
                                          t.cz.get(), t.cr.get(), t.Mf.get());
     }
@@ -369,10 +500,14 @@ void far_field_mixed(Tree& t, int mode, int mask, hipStream_t st) {
     for (int l = 1; l < t.depth; ++l) {
       const int b = t.level_off[l], e = t.level_off[l + 1];
       const dim3 g(nblk(e - b, 64), (LS + kChunk - 1) / kChunk);
-      if (top64 && l - 1 < kTopLevels)
-        k_l2l_top<P><<<g, 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
-                                       t.Lf.get(), t.Ltop.get(), ntop);
-      else
+      if (top64 && l - 1 < kTopLevels) {
+        if constexpr (SO == 0)
+          k_l2l_top_wave<P><<<e - b, 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(),
+                                                  t.cr.get(), t.Lf.get(), t.Ltop.get(), ntop);
+        else
+          k_l2l_top<P><<<g, 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
+                                         t.Lf.get(), t.Ltop.get(), ntop);
+      } else
         k_l2l_mixed<P><<<g, 64, 0, st>>>(b, e, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(),
                                          t.Lf.get());
     }
