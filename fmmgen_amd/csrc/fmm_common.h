@@ -114,11 +114,12 @@ struct M2LSchedule {
   // per evaluation: the D vector of every key (FP64 plain, normalised FP32 mixed)
   DevBuf<double> Dtab64;
   DevBuf<float> Dtab32;
-  // (step, lane) -> D slot table of the GEMM plan, uploaded once per order
+  // (step, lane) -> D slot tables of the GEMM plans, uploaded once per
+  // order and block size (4 or 8 waves)
   static constexpr int kMaxOrder = 16;
-  DevBuf<unsigned char> idxtab[kMaxOrder];
+  DevBuf<unsigned> idxtab[kMaxOrder][2];
   DevBuf<short> idxtab32[kMaxOrder];
-  bool idx_ready[kMaxOrder] = {false}, idx32_ready[kMaxOrder] = {false};
+  bool idx_ready[kMaxOrder][2] = {}, idx32_ready[kMaxOrder] = {false};
   // learned order: chunks sorted by their tile's order, and the start of
   // every order's range (kMaxOrder + 1 entries)
   DevBuf<int> class_chunks;

@@ -281,6 +281,19 @@ struct WavePlanRR {
 template <int P, int SO, int W, int WV>
 using WavePlanFor = std::conditional_t<(P >= 10), WavePlan<P, SO, W, WV>, WavePlanRR<P, SO, W, WV>>;
 
+// The index table holds, per wave, its steps' 8-bit D offsets packed four
+// to a 32-bit word (one LDS read serves four steps): word d of lane l of
+// wave wv sits at wave_words<..., wv>() + 64 d + l and holds steps
+// 4d .. 4d + 3 of the wave's order in bytes 0 .. 3.
+template <int P, int SO, int W, int WV>
+constexpr int wave_words() {
+  if constexpr (WV == 0) {
+    return 0;
+  } else {
+    return wave_words<P, SO, W, WV - 1>() + 64 * ((WavePlanFor<P, SO, W, WV - 1>::tab.n + 3) / 4);
+  }
+}
+
 // Waves per block: the LDS tables of a chunk are shared by W waves.
 template <int P, int SO, typename T>
 constexpr int m2l_waves() {
@@ -309,7 +322,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
                                                      const int* __restrict__ group_key,
                                                      const int* __restrict__ group_src,
                                                      const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
-                                                     const unsigned char* __restrict__ idxtab,
+                                                     const unsigned* __restrict__ idxtab,
                                                      const signed char* __restrict__ tile_order,
                                                      const int* __restrict__ chunk_list, int mstride, int lstride,
                                                      const T* __restrict__ M, T* __restrict__ L) {
@@ -320,7 +333,8 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   constexpr int CPW = kTile / W;         // columns staged by each wave
   constexpr int DJ = (NJ + W - 1) / W;   // D segments staged by each wave
   using V4 = typename Vec4<T>::type;
-  __shared__ unsigned char sidx[NST * 64];  // offsets from each step's base slot
+  constexpr int NWORDS = wave_words<P, SO, W, W>();
+  __shared__ unsigned sidx[NWORDS];  // offsets from each step's base slot, four steps a word
   __shared__ T sD[MS + 256];                 // slots MS .. MS + 255 stay zero
   __shared__ T sB[kTile * MSP + 4];          // tail slots stay zero
   __shared__ signed char sdeg[MS];
@@ -329,7 +343,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   if (ci >= nchunks) return;  // uniform over the block
   const int chunk = chunk_list ? chunk_list[ci] : ci;
   const int tile = chunk_tile[chunk];
-  for (int i = tid; i < NST * 64; i += 64 * W) sidx[i] = idxtab[i];
+  for (int i = tid; i < NWORDS; i += 64 * W) sidx[i] = idxtab[i];
   for (int i = tid; i < 256; i += 64 * W) sD[MS + i] = T(0);
   for (int i = tid; i < kTile * MSP + 4; i += 64 * W) sB[i] = T(0);
   for (int j = tid; j < MS; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
@@ -396,7 +410,9 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
       // MFMAs (a tile of a lower order than P has its D entries of higher
       // degree zeroed; the learned evaluation launches each tile order with
       // the kernel of that order, so no step is wasted there)
-      constexpr int kBatch = 16;
+      constexpr int kBatch = 16;  // a multiple of 4: a batch starts a packed word
+      constexpr int WW = wave_words<P, SO, W, wv>();
+      unsigned word = 0;
       static_for<(NSW + kBatch - 1) / kBatch>([&](auto bb) {
         {
           T av[kBatch], bv[kBatch];
@@ -406,7 +422,8 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
               constexpr int s = WP::tab.s[i];
               constexpr int K = Plan::tab.sK[s], I = Plan::tab.sI[s];
               constexpr int base = step_base<P, SO>(I, K);
-              const int off = sidx[s * 64 + lane];
+              if constexpr (i % 4 == 0) word = sidx[WW + 64 * (i / 4) + lane];
+              const int off = (int)((word >> (8 * (i % 4))) & 255u);
               av[u] = sD[base + off];
               // an empty entry (offset 255) reads a zero slot when the
               // step's base is within 255 of MS, else it is masked here
@@ -662,22 +679,35 @@ std::vector<short> m2l_index_table32() {
   return t;
 }
 
-// Host side: the (step, lane) table of the plan: 8-bit offsets of the D
-// slot from the step's base slot, 255 for an empty entry.
-template <int P, int SO>
-std::vector<unsigned char> m2l_index_table() {
-  using Plan = M2LPlan<P, SO>;
-  std::vector<unsigned char> t((size_t)Plan::NSTEP * 64, 255);
-  for (int s = 0; s < Plan::NSTEP; ++s) {
-    const int I = Plan::tab.sI[s], K = Plan::tab.sK[s];
-    const int base = step_base<P, SO>(I, K);
-    for (int l = 0; l < 64; ++l) {
-      const int v = step_lane_slot<P, SO>(I, K, l);
-      if (v < 0) continue;
-      if (v - base > 254) throw Error{kUnsupported, "M2L step spans more than 255 D slots"};
-      t[(size_t)s * 64 + l] = (unsigned char)(v - base);
+// Host side: the index table of a W-wave block (layout at wave_words):
+// 8-bit offsets of each lane's D slot from its step's base slot, 255 for
+// an empty entry.
+template <int P, int SO, int W, int WV>
+void fill_wave_index(std::vector<unsigned>& t) {
+  if constexpr (WV < W) {
+    using WP = WavePlanFor<P, SO, W, WV>;
+    using Plan = M2LPlan<P, SO>;
+    const size_t w0 = (size_t)wave_words<P, SO, W, WV>();
+    for (int i = 0; i < WP::tab.n; ++i) {
+      const int st = WP::tab.s[i];
+      const int I = Plan::tab.sI[st], K = Plan::tab.sK[st];
+      const int base = step_base<P, SO>(I, K);
+      for (int l = 0; l < 64; ++l) {
+        const int v = step_lane_slot<P, SO>(I, K, l);
+        if (v >= 0 && v - base > 254) throw Error{kUnsupported, "M2L step spans more than 255 D slots"};
+        const unsigned b = v < 0 ? 255u : (unsigned)(v - base);
+        unsigned& word = t[w0 + 64 * (size_t)(i / 4) + l];
+        word = (word & ~(255u << (8 * (i % 4)))) | (b << (8 * (i % 4)));
+      }
     }
+    fill_wave_index<P, SO, W, WV + 1>(t);
   }
+}
+
+template <int P, int SO, int W>
+std::vector<unsigned> m2l_index_table() {
+  std::vector<unsigned> t((size_t)wave_words<P, SO, W, W>(), 0xffffffffu);
+  fill_wave_index<P, SO, W, 0>(t);
   return t;
 }
 

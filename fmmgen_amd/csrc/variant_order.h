@@ -116,17 +116,20 @@ __global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const doubl
   for (int j = 0; j < MS; ++j) Dtab[(size_t)k * MS + j] = (T)D[j];
 }
 
-template <int P>
-const unsigned char* ensure_index_table(Tree& t, hipStream_t st) {
+template <int P, int W>
+const unsigned* ensure_index_table(Tree& t, hipStream_t st) {
+  static_assert(W == 4 || W == 8, "blocks of 4 or 8 waves");
   M2LSchedule& S = t.sched;
-  if (!S.idx_ready[P]) {
-    const std::vector<unsigned char> tab = m2l_index_table<P, SO>();
-    S.idxtab[P].alloc(tab.size());
-    FMM_HIP(hipMemcpyAsync(S.idxtab[P].get(), tab.data(), tab.size(), hipMemcpyHostToDevice, st));
+  const int wi = W == 8 ? 1 : 0;
+  if (!S.idx_ready[P][wi]) {
+    const std::vector<unsigned> tab = m2l_index_table<P, SO, W>();
+    S.idxtab[P][wi].alloc(tab.size());
+    FMM_HIP(hipMemcpyAsync(S.idxtab[P][wi].get(), tab.data(), sizeof(unsigned) * tab.size(),
+                           hipMemcpyHostToDevice, st));
     FMM_HIP(hipStreamSynchronize(st));
-    S.idx_ready[P] = true;
+    S.idx_ready[P][wi] = true;
   }
-  return S.idxtab[P].get();
+  return S.idxtab[P][wi].get();
 }
 
 template <int P>
@@ -146,11 +149,11 @@ template <int P>
 void m2l_plain(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (S.ngroups == 0) return;
-  const unsigned char* idx = ensure_index_table<P>(t, st);
+  constexpr int W = m2l_waves<P, SO, double>();
+  const unsigned* idx = ensure_index_table<P, W>(t, st);
   S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
   const int grid = ((S.nchunks + 7) / 8) * 8;
-  constexpr int W = m2l_waves<P, SO, double>();
   k_m2l_gemm<P, SO, double, false, W><<<grid, 64 * W, 0, st>>>(
       S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
       S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(), idx, t.tile_order_ptr(), nullptr, Op<P>::MS,
@@ -254,8 +257,8 @@ void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hi
                                              ms, ls, t.Mf.get(), t.Lf.get());
     return;
   }
-  const unsigned char* idx = ensure_index_table<Q>(t, st);
   constexpr int W = m2l_waves<Q, SO, float>();
+  const unsigned* idx = ensure_index_table<Q, W>(t, st);
   k_m2l_gemm<Q, SO, float, true, W><<<grid, 64 * W, 0, st>>>(
       nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
       S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks, ms, ls, t.Mf.get(),
