@@ -279,7 +279,7 @@ struct WavePlanRR {
 
 // The plan a kernel of order P uses (measured choice).
 template <int P, int SO, int W, int WV>
-using WavePlanFor = std::conditional_t<(P >= 10), WavePlan<P, SO, W, WV>, WavePlanRR<P, SO, W, WV>>;
+using WavePlanFor = std::conditional_t<(P >= 6), WavePlan<P, SO, W, WV>, WavePlanRR<P, SO, W, WV>>;
 
 // The index table holds, per wave, its steps' 8-bit D offsets packed four
 // to a 32-bit word (one LDS read serves four steps): word d of lane l of
