@@ -62,47 +62,58 @@ def mac_flops_m2l(p, so):
     return 2 * mac_count_m2l(p, so)
 
 
-def cpu_baseline(cfg, budget_s=30.0):
+def cpu_baseline(cfg):
     """The oracle (C++/OpenMP restatement of the example) on a bounded sample.
 
-    Sample: the same distribution, order, theta and ncrit at n = min(N, 2^20)
-    (2^18 at order 8 and above, to keep the sample to tens of seconds), one
-    evaluation timed."""
+    Sample: the same distribution, order, theta and ncrit at n = min(N, 2^20),
+    one evaluation timed (10-20 s on the host's cores at order 10), with the
+    example's run schedule OMP_SCHEDULE=dynamic,16 (example/run.sh:5) for its
+    schedule(runtime) loops."""
     from oracle import oracle as O
-    import ctypes
     so = cfg["so"]
     outsz = 4 if so == 0 else 3
-    n = min(cfg["n"], (1 << 18) if cfg["order"] >= 8 else (1 << 20))
+    n = min(cfg["n"], 1 << 20)
     sub = dict(cfg, n=n)
     pos, S = make_inputs(sub, seed=2)
+    L = O.lib(so, outsz)
+    L.oracle_set_runtime_schedule(16)
     t0 = time.perf_counter()
     T = O.OracleTree(pos, S, cfg["ncrit"], cfg["order"], cfg["theta"], source_order=so, outputsize=outsz)
     t1 = time.perf_counter()
     T.compute_field_fmm()
     t2 = time.perf_counter()
-    cores = O.lib(so, outsz).oracle_num_threads()
+    cores = L.oracle_num_threads()
     return {"value": n / (t2 - t1), "unit": "particle-evals/s", "cores": int(cores), "kind": "port",
             "sample": f"oracle compute_field_fmm, n={n} {cfg['dist']}, p={cfg['order']}, "
-                      f"theta={cfg['theta']}, ncrit={cfg['ncrit']}, one evaluation "
+                      f"theta={cfg['theta']}, ncrit={cfg['ncrit']}, OMP_SCHEDULE dynamic,16, one evaluation "
                       f"({t2 - t1:.1f} s; tree build {t1 - t0:.1f} s excluded)"}
 
 
 def timed(tree, F, mode, steps, warmup):
-    """(ms per evaluation, per-phase ms per evaluation) of ``steps`` timed evaluations."""
+    """Times ``steps`` evaluations after ``warmup`` untimed ones.
+
+    Returns (median ms per evaluation from hipEvents recorded around each
+    evaluation on the evaluation stream, mean wall-clock ms per evaluation
+    of the synchronised run, per-phase ms per evaluation)."""
     import torch
     for _ in range(warmup):
         tree.compute_field_fmm(F, mode=mode)
     torch.cuda.synchronize()
     tree.timing(True)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    ev[0].record()
+    for k in range(steps):
         tree.compute_field_fmm(F, mode=mode)
+        ev[k + 1].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     phases = tree.timing_read()
     tree.timing(False)
-    return (t1 - t0) * 1e3 / steps, {k: v[0] / steps for k, v in phases.items()}
+    per = sorted(ev[k].elapsed_time(ev[k + 1]) for k in range(steps))
+    med = per[steps // 2] if steps % 2 else 0.5 * (per[steps // 2 - 1] + per[steps // 2])
+    return med, (t1 - t0) * 1e3 / steps, {k: v[0] / steps for k, v in phases.items()}
 
 
 def field_error(F, Fref, tg, so):
@@ -112,6 +123,20 @@ def field_error(F, Fref, tg, so):
     ferr = (torch.linalg.norm(Fs[:, fo:] - Fref[:, fo:]) / torch.linalg.norm(Fref[:, fo:])).item()
     perr = (torch.linalg.norm(Fs[:, 0] - Fref[:, 0]) / torch.linalg.norm(Fref[:, 0])).item() if so == 0 else None
     return ferr, perr
+
+
+def error_detail(F, Fref, tg, so):
+    """Per-target and per-component error figures at the sampled targets:
+    the 99th percentile of |dF_i| / |F_i| (field vectors) and the example's
+    per-component mean |(exact - approx) / exact| (example/main.cpp:166-188)."""
+    import torch
+    Fs = F[tg]
+    fo = 1 if so == 0 else 0
+    d = torch.linalg.norm(Fs[:, fo:] - Fref[:, fo:], dim=1) / torch.linalg.norm(Fref[:, fo:], dim=1)
+    mean_abs_rel = ((Fref - Fs) / Fref).abs().mean(dim=0)
+    return {"p99_target_field_rel_err": torch.quantile(d, 0.99).item(),
+            "max_target_field_rel_err": d.max().item(),
+            "mean_abs_rel_err_per_component": [float(v) for v in mean_abs_rel.tolist()]}
 
 
 def roofline_of(phases, info, order, so, mode, m2l_macs=None, config="c4"):
@@ -206,11 +231,13 @@ def main():
                 break
         pfix = pfix or 12
         tree.set_order(pfix)
-        ms_plain, ph_plain = timed(tree, F, "plain", args.steps, args.warmup)
+        ms_plain, wall_plain, ph_plain = timed(tree, F, "plain", args.steps, args.warmup)
         e_plain, _ = field_error(F, Fref, tg, so)
-        print(f"# plain FP64 order {pfix}: {ms_plain:.3f} ms/eval", file=sys.stderr, flush=True)
-        extra["plain_f64"] = {"order": pfix, "ms_per_step": ms_plain, "value": n / (ms_plain * 1e-3),
-                              "field_rel_err": e_plain, "phase_ms": ph_plain}
+        print(f"# plain FP64 order {pfix}: {ms_plain:.3f} ms/eval (median), {wall_plain:.3f} (wall mean)",
+              file=sys.stderr, flush=True)
+        extra["plain_f64"] = {"order": pfix, "ms_per_step": ms_plain, "ms_per_step_wall": wall_plain,
+                              "value": n / (ms_plain * 1e-3), "field_rel_err": e_plain,
+                              "errors": error_detail(F, Fref, tg, so), "phase_ms": ph_plain}
         tree.set_order(full)
         mode = "learned"
     else:
@@ -220,10 +247,11 @@ def main():
         params, margin, _ = LN.load(args.model or LN.MODEL_PATH)
         tree.set_model(params, args.tol, margin)
         extra["model_margin"] = margin
-    ms_per_step, avg = timed(tree, F, mode, args.steps, args.warmup)
-    print(f"# {mode}: {ms_per_step:.3f} ms/eval; phases (ms/eval): "
+    ms_per_step, ms_wall, avg = timed(tree, F, mode, args.steps, args.warmup)
+    print(f"# {mode}: {ms_per_step:.3f} ms/eval (median), {ms_wall:.3f} (wall mean); phases (ms/eval): "
           + ", ".join(f"{k} {v:.3f}" for k, v in avg.items()), file=sys.stderr, flush=True)
     ferr, perr = field_error(F, Fref, tg, so)
+    errs = error_detail(F, Fref, tg, so)
     stats = tree.learned_stats() if learned else None
     if stats:
         extra["learned"] = stats
@@ -247,11 +275,12 @@ def main():
     out = {
         "metric": "FMM particle-evals/sec at 1e-6 rel field err, mixed+learned vs FP64 plain",
         "value": value, "unit": "particle-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "ms_per_step": ms_per_step, "ms_per_step_wall": ms_wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": dtype, "data": "synthetic",
         "config": {"workload": workload, "mode": mode, "n": n, "order": full if learned else cfg["order"],
                    "theta": cfg["theta"], "ncrit": cfg["ncrit"]},
         "field_rel_err": ferr, "potential_rel_err": perr, "meets_1e-6": bool(ferr <= 1e-6),
+        "errors": errs, "timing": "median of hipEvent-timed evaluations on the evaluation stream",
         "tree_build_ms": build_ms, "phase_ms": avg, "tree": info,
         "roofline": roofline, "cpu_baseline": base,
     }

@@ -32,7 +32,20 @@ def lib(source_order=0, outputsize=4):
         return _libs[key]
     if key not in _VARIANTS:
         raise ValueError(f"no oracle variant for source_order={source_order}, outputsize={outputsize}")
-    path = os.path.join(BUILD, _VARIANTS[key])
+    _libs[key] = _load(os.path.join(BUILD, _VARIANTS[key]))
+    return _libs[key]
+
+
+def mutant_lib(k):
+    """The charge oracle (potential + field) built with planted tree/list
+    mistake k (ORACLE_MUTANT in fmm_oracle.cpp), for tests of the tests."""
+    key = ("mutant", k)
+    if key not in _libs:
+        _libs[key] = _load(os.path.join(BUILD, f"mutant_{k}.so"))
+    return _libs[key]
+
+
+def _load(path):
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
     L = ctypes.CDLL(path)
@@ -56,7 +69,7 @@ def lib(source_order=0, outputsize=4):
     for nm in ("oracle_op_M2M", "oracle_op_M2L", "oracle_op_L2L", "oracle_op_L2P", "oracle_op_M2P"):
         getattr(L, nm).argtypes = [ctypes.c_double] * 3 + [c_double_p, c_double_p, ctypes.c_int]
     L.oracle_op_P2P.argtypes = [ctypes.c_double] * 3 + [c_double_p, c_double_p]
-    _libs[key] = L
+    L.oracle_set_runtime_schedule.argtypes = [ctypes.c_int]
     return L
 
 
@@ -71,12 +84,12 @@ def _pi(a):
 class OracleTree:
     """build_tree + Tree::compute_field_* of the example, on the CPU."""
 
-    def __init__(self, pos, S, ncrit, order, theta, source_order=0, outputsize=4):
+    def __init__(self, pos, S, ncrit, order, theta, source_order=0, outputsize=4, library=None):
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         self.n = self.pos.shape[0]
         self.S = np.ascontiguousarray(S, dtype=np.float64).reshape(self.n, -1)
         self.outputsize = outputsize
-        self.lib = lib(source_order, outputsize)
+        self.lib = library if library is not None else lib(source_order, outputsize)
         self.h = self.lib.oracle_build_tree(_p(self.pos), _p(self.S), self.n, ncrit, order, float(theta))
         self.ncrit, self.order, self.theta = ncrit, order, theta
 

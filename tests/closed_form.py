@@ -41,14 +41,66 @@ def mslot(so):
 
 
 @lru_cache(maxsize=None)
-def derivs(x, maxdeg):
-    """D_n = d^n (1/R) at x for |n| <= maxdeg (fmmgen/expansions.py:108-141)."""
+def derivs_numdiff(x, maxdeg):
+    """D_n = d^n (1/R) at x for |n| <= maxdeg (fmmgen/expansions.py:108-141),
+    by mpmath's numerical differentiation (slow above degree 8)."""
     X = tuple(mp.mpf(v) for v in x)
     f = lambda a, b, c: 1 / mp.sqrt(a * a + b * b + c * c)  # noqa: E731
     out = {}
     for n in monomials(maxdeg, 0):
         out[n] = mp.diff(f, X, n)
     return out
+
+
+def _polymul(a, b, maxdeg):
+    out = {}
+    for ka, va in a.items():
+        da = _deg(ka)
+        for kb, vb in b.items():
+            if da + _deg(kb) > maxdeg:
+                continue
+            k = _add(ka, kb)
+            out[k] = out.get(k, 0) + va * vb
+    return out
+
+
+@lru_cache(maxsize=None)
+def derivs_series(x, maxdeg):
+    """D_n = d^n (1/R) at x for |n| <= maxdeg from the binomial series of
+    1/|x + h| = (1/r) (1 + u)^(-1/2), u = (2 x.h + h.h) / r^2:
+    D_n = n! [h^n] sum_k C(-1/2, k) u^k / r.  Exact rational polynomial
+    arithmetic in h, one mpmath square root at the end; shares nothing with
+    numerical differentiation or the generator's recurrence."""
+    from fractions import Fraction
+    X = [Fraction(v) for v in x]
+    r2 = X[0] ** 2 + X[1] ** 2 + X[2] ** 2
+    u = {}
+    for a in range(3):
+        e = [0, 0, 0]
+        e[a] = 1
+        u[tuple(e)] = 2 * X[a] / r2
+        e[a] = 2
+        u[tuple(e)] = Fraction(1) / r2
+    total = {(0, 0, 0): Fraction(1)}
+    uk = {(0, 0, 0): Fraction(1)}
+    binom = Fraction(1)
+    for k in range(1, maxdeg + 1):  # u^k has lowest degree k
+        uk = _polymul(uk, u, maxdeg)
+        binom = binom * (Fraction(-1, 2) - (k - 1)) / k
+        for key, v in uk.items():
+            total[key] = total.get(key, 0) + binom * v
+    rinv = 1 / mp.sqrt(mp.mpf(r2.numerator) / r2.denominator)
+    out = {}
+    for n in monomials(maxdeg, 0):
+        c = total.get(n, Fraction(0)) * _f(n)
+        out[n] = mp.mpf(c.numerator) / c.denominator * rinv
+    return out
+
+
+def derivs(x, maxdeg):
+    """D_n = d^n (1/R) at x for |n| <= maxdeg: the binomial series (checked
+    against numerical differentiation in the tests)."""
+    return derivs_series(tuple(float(v) for v in x), maxdeg)
 
 
 def P2M(x, q, p, so):
