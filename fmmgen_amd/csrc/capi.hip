@@ -196,7 +196,7 @@ int fmm_tree_info(fmm_tree_t tree, long long* info) {
     info[11] = t.sched.ngroups;
     info[12] = t.sched.nkeys;
     info[13] = t.n_p2p_items;
-    info[14] = 0;
+    info[14] = t.sched.nchunks;
     info[15] = 0;
   });
 }
@@ -329,8 +329,8 @@ int fmm_export_expansions(fmm_tree_t tree, double* h_M, double* h_L) {
     const Tree& t = tree->t;
     FMM_HIP(hipDeviceSynchronize());
     const size_t ms = (size_t)msize(t.order, t.so) * t.ncells, ls = (size_t)lsize(t.order, t.so) * t.ncells;
-    FMM_CHECK(!h_M || t.M.n >= ms, kInvalidArgument, "no FP64 expansions: evaluate first");
-    FMM_CHECK(!h_L || t.L.n >= ls, kInvalidArgument, "no FP64 expansions: evaluate first");
+    FMM_CHECK(t.ml_order == t.order && t.M.n >= ms && t.L.n >= ls, kInvalidArgument,
+              "no FP64 expansions at the tree's order: the last evaluation was not a plain FP64 one at it");
     if (h_M) FMM_HIP(hipMemcpy(h_M, t.M.get(), sizeof(double) * ms, hipMemcpyDeviceToHost));
     if (h_L) FMM_HIP(hipMemcpy(h_L, t.L.get(), sizeof(double) * ls, hipMemcpyDeviceToHost));
   });
@@ -392,8 +392,8 @@ int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream) {
   return guarded([&] {
     FMM_CHECK(tree && d_feat, kInvalidArgument, "NULL argument");
     Tree& t = tree->t;
-    FMM_CHECK(t.pS.get() != nullptr, kInvalidArgument, "no evaluation has run on this tree");
     hipStream_t st = static_cast<hipStream_t>(stream);
+    tree->v->gather_sources(t, st);  // the current strengths (set_sources may have rebound them)
     tree->v->field_scale(t, st);
     leaf_features(t, sourcesize(t.so), st);
     if (t.nleaves > 0)
@@ -407,9 +407,11 @@ int fmm_tree_field_scale(fmm_tree_t tree, double* h_scale, void* stream) {
   return guarded([&] {
     FMM_CHECK(tree && h_scale, kInvalidArgument, "NULL argument");
     Tree& t = tree->t;
-    FMM_CHECK(t.pS.get() != nullptr, kInvalidArgument, "no evaluation has run on this tree");
-    tree->v->field_scale(t, static_cast<hipStream_t>(stream));
-    *h_scale = t.judge_scale;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    tree->v->gather_sources(t, st);
+    tree->v->field_scale(t, st);
+    FMM_HIP(hipMemcpyAsync(h_scale, t.lw.get() + kLwScale, sizeof(double), hipMemcpyDeviceToHost, st));
+    FMM_HIP(hipStreamSynchronize(st));
   });
 }
 
@@ -437,6 +439,8 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
     FMM_CHECK(tree && d_params, kInvalidArgument, "NULL argument");
     FMM_CHECK(tol > 0 && tol < 1, kInvalidArgument, "tolerance must be in (0, 1)");
     Tree& t = tree->t;
+    // no evaluation on any stream may still read the buffers re-formed here
+    FMM_HIP(hipDeviceSynchronize());
     t.mlp_half.alloc(kMlpParams);
     mlp_to_half(d_params, t.mlp_half.get(), nullptr);
     FMM_HIP(hipStreamSynchronize(nullptr));
@@ -460,7 +464,20 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
 int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats) {
   return guarded([&] {
     FMM_CHECK(tree && h_stats, kInvalidArgument, "NULL argument");
-    for (int k = 0; k < 10; ++k) h_stats[k] = tree->t.lstats[k];
+    Tree& t = tree->t;
+    if (!t.lstats_ready && t.lw.get() && t.cell_q.get()) {  // order statistics of the last learned evaluation
+      FMM_HIP(hipDeviceSynchronize());
+      t.use_tile_order = true;
+      t.lstats[5] = m2l_mac_fraction(t, t.order, nullptr, &t.lstats[6]);
+      t.use_tile_order = false;
+      double lw[kLwCount];
+      FMM_HIP(hipMemcpy(lw, t.lw.get(), sizeof(lw), hipMemcpyDeviceToHost));
+      t.lstats[7] = lw[kLwEffFinal];
+      t.lstats[8] = lw[kLwScale];
+      t.lstats[9] = m2l_cell_macs(t, nullptr);
+      t.lstats_ready = true;
+    }
+    for (int k = 0; k < 10; ++k) h_stats[k] = t.lstats[k];
   });
 }
 

@@ -34,6 +34,7 @@ struct Variant {
   void (*evaluate_learned)(Tree&, int mode, double* F, hipStream_t);
   void (*field_scale)(Tree&, hipStream_t);
   void (*predict_cell_orders)(Tree&, hipStream_t);
+  void (*gather_sources)(Tree&, hipStream_t);
   void (*op_batch)(int op, int order, int n, const double* xyz, const double* in, int in_stride, double* out,
                    int out_stride, hipStream_t);
   void (*direct)(int n, const double* pos, const double* S, const long long* targets, int nt, double* F,

@@ -101,6 +101,7 @@ enum PhaseTimerId : int {
 constexpr int kTile = 16;        // target cells per M2L tile (one MFMA column block)
 
 struct M2LSchedule {
+  static constexpr int tile_size = kTile;
   int ntiles = 0;
   int ngroups = 0;               // (tile, offset) groups
   int nkeys = 0;                 // distinct (target level, source level, offset) keys
@@ -118,11 +119,13 @@ struct M2LSchedule {
   // order and block size (4 or 8 waves)
   static constexpr int kMaxOrder = 16;
   DevBuf<unsigned> idxtab[kMaxOrder][2];
-  DevBuf<short> idxtab32[kMaxOrder];
+  DevBuf<unsigned> idxtab32[kMaxOrder];
   bool idx_ready[kMaxOrder][2] = {}, idx32_ready[kMaxOrder] = {false};
   // learned order: chunks sorted by their tile's order, and the start of
   // every order's range (kMaxOrder + 1 entries)
   DevBuf<int> class_chunks;
+  DevBuf<int> class_start;               // [kMaxOrder + 1] first chunk of every order (-1: none)
+  DevBuf<unsigned long long> class_keys; // sort keys of the order classes
   DevBuf<long long> tile_pairs;  // M2L pairs per tile (statistics)
   bool tile_pairs_ready = false;
   DevBuf<int> cell_tile;         // cell -> its M2L tile (-1: none)
@@ -184,6 +187,10 @@ struct Tree {
   // order of its 16 target cells; the tile's groups are truncated there)
   DevBuf<float> leaf_feat;         // [nleaves * kMlpF]
   DevBuf<int> leaf_q, cell_q;      // [nleaves], [ncells]
+  DevBuf<int> leaf_q2, cell_q2a;   // the second allocation's orders
+  DevBuf<signed char> tile_order2;
+  DevBuf<double> lw;               // device scalars of the order selection (learned.h)
+  DevBuf<double> lam_sums;         // multiplier search sums
   DevBuf<int> cell_eff;            // [ncells] effective order (lowest tile order on the chain)
   DevBuf<int> tile_hint;           // [ncells] order hint of the M2L tiling (set with a model)
   bool use_tile_hint = false;
@@ -191,7 +198,6 @@ struct Tree {
   DevBuf<float> leaf_y;            // [nleaves * kMlpO] network outputs
   DevBuf<double> cell_qabs, cell_q2;  // [ncells] sum of |S| and of |S|^2 below the cell
   DevBuf<double> cell_acc;            // [ncells * 11] list sums, accumulated down the tree
-  double judge_scale = 0.0;        // field scale of the judge sample (network output scale)
 
   DevBuf<signed char> tile_order;  // [ntiles]
   bool use_tile_order = false;
@@ -208,9 +214,14 @@ struct Tree {
   DevBuf<double> judge_Fd, judge_acc;  // [judge_n * OUT], [4]
   DevBuf<int> judge_fail;              // [1 + ncells] failing count, per cell flag
   double lstats[10] = {0};
+  bool lstats_ready = false;       // lstats[5..9] computed for the last learned evaluation
+  int class_start_h[M2LSchedule::kMaxOrder + 1] = {0};  // host copy: first work item of every supertile order
 
-  // expansions, [ncells * Msize] and [ncells * Lsize] in FP64
+  // expansions, [ncells * Msize] and [ncells * Lsize] in FP64; ml_order =
+  // the order of the FP64 evaluation that filled them (-1: the last
+  // evaluation was mixed or learned)
   DevBuf<double> M, L;
+  int ml_order = -1;
   // mixed precision: normalised FP32 expansions, FP64 copies of the top levels
   DevBuf<float> Mf, Lf;
   DevBuf<double> Mtop, Ltop;
@@ -224,9 +235,25 @@ struct Tree {
   bool timing = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
   std::vector<hipEvent_t> event_pool;
+
+  DevBuf<int> p2p_next;            // work counters of the persistent near-field kernel
+  int num_cu = 0;                  // compute units of the device (queried once)
   double phase_ms[kNumPhaseTimers] = {0};
   long long phase_count[kNumPhaseTimers] = {0};
 };
+
+// Blocks of the persistent near-field kernel: kP2PWavesPerSimd one-wave
+// blocks per SIMD of every compute unit.
+constexpr int kP2PWavesPerSimd = 8;
+inline int p2p_grid(Tree& t) {
+  if (t.num_cu == 0) {
+    int dev = 0;
+    FMM_HIP(hipGetDevice(&dev));
+    FMM_HIP(hipDeviceGetAttribute(&t.num_cu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (t.num_cu <= 0) t.num_cu = 256;
+  }
+  return t.num_cu * 4 * kP2PWavesPerSimd;
+}
 
 // RAII helper: records a start/stop event pair for one phase when timing
 // is on.

@@ -119,9 +119,10 @@ __global__ void k_acc_down(int beg, int end, const int* __restrict__ parent, dou
 }
 
 __global__ void k_leaf_feat(int nleaves, const int* __restrict__ leaves, const int* __restrict__ pcount,
-                            const double* __restrict__ acc, double fn, float* __restrict__ feat) {
+                            const double* __restrict__ acc, const double* __restrict__ fnp, float* __restrict__ feat) {
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   if (li >= nleaves) return;
+  const double fn = *fnp;
   const int c = leaves[li];
   const double* a = acc + (size_t)c * kAcc;
   float* f = feat + (size_t)li * kMlpF;
@@ -157,13 +158,13 @@ __global__ void k_leaf_to_cell(int nleaves, const int* __restrict__ leaves, cons
   if (li < nleaves) cq[leaves[li]] = lq[li];
 }
 
-__global__ void k_tile_order(int ntiles, const int* __restrict__ tile_cells, const int* __restrict__ cq,
+__global__ void k_tile_order(int ntiles, int ts, const int* __restrict__ tile_cells, const int* __restrict__ cq,
                              signed char* __restrict__ to) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= ntiles) return;
   int q = 0;
-  for (int k = 0; k < kTile; ++k) {
-    const int c = tile_cells[t * kTile + k];
+  for (int k = 0; k < ts; ++k) {
+    const int c = tile_cells[(size_t)t * ts + k];
     if (c >= 0) q = max(q, cq[c]);
   }
   to[t] = (signed char)q;
@@ -246,9 +247,11 @@ __global__ void k_class_split(int nch, const unsigned long long* __restrict__ ke
 // e_l(q) = 10^(Y_l(q) + margin) its predicted rms error relative to the rms
 // field, n_l its particles (2^f6) and w_l the M2L pairs on its ancestor
 // chain (10^f5 - 1), the pairs whose order it raises.  Adds sum n_l e_l^2.
-__global__ void k_alloc(int nl, const float* __restrict__ feat, const float* __restrict__ Y, double loglam,
-                        float margin, int qmin, int qmax, int* __restrict__ q_out, double* __restrict__ err_sum) {
+__global__ void k_alloc(int nl, const float* __restrict__ feat, const float* __restrict__ Y, const double* lw,
+                        float margin, int qmin, int qmax, int* __restrict__ q_out, double* err_sum, int gated) {
   __shared__ double red[256];
+  if (gated && lw[kLwRealloc] == 0.0) return;  // uniform: the second allocation is not needed
+  const double loglam = lw[kLwLogLam];
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
   double contrib = 0.0;
   if (li < nl) {
@@ -297,11 +300,11 @@ __global__ void k_cell_macs(int ncells, const int* __restrict__ off, const int* 
 }
 
 // cell -> M2L tile of the cell (-1: no M2L targets), once per tree
-__global__ void k_cell_tile(int ntiles, const int* __restrict__ tile_cells, int* __restrict__ cell_tile) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ntiles * kTile) return;
+__global__ void k_cell_tile(int ntiles, int ts, const int* __restrict__ tile_cells, int* __restrict__ cell_tile) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= (long long)ntiles * ts) return;
   const int c = tile_cells[i];
-  if (c >= 0) cell_tile[c] = i / kTile;
+  if (c >= 0) cell_tile[c] = (int)(i / ts);
 }
 
 // effective order of every cell: the lowest tile order over its ancestor
@@ -341,9 +344,12 @@ __global__ void k_eff_err(int nl, const int* __restrict__ leaves, const float* _
 
 // The error sums of kLam multipliers at once: lam_k = 10^(lo + k step).
 constexpr int kLam = 64;
-__global__ void k_alloc_multi(int nl, const float* __restrict__ feat, const float* __restrict__ Y, double lo,
-                              double step, float margin, int qmin, int qmax, double* __restrict__ sums) {
+__global__ void k_alloc_multi(int nl, const float* __restrict__ feat, const float* __restrict__ Y,
+                              const double* __restrict__ lw, float margin, int qmin, int qmax,
+                              double* __restrict__ sums, int gated) {
   __shared__ float sh[kLam];
+  if (lw[kLwSkip] != 0.0 || (gated && lw[kLwRealloc] == 0.0)) return;  // uniform over the grid
+  const double lo = lw[kLwLo], step = lw[kLwStep];
   for (int k = threadIdx.x; k < kLam; k += blockDim.x) sh[k] = 0.f;
   __syncthreads();
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,34 +384,184 @@ __global__ void k_alloc_multi(int nl, const float* __restrict__ feat, const floa
   for (int k = threadIdx.x; k < kLam; k += blockDim.x) atomicAdd(&sums[k], (double)sh[k]);
 }
 
+// Start of a multiplier search: target sum n_l e_l^2 = tol^2 n, coarse
+// grid 10^-10 .. 10^40; tol from lw[tol_slot].
+__global__ void k_lam_init(double* __restrict__ lw, int tol_slot, double n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double tol = lw[tol_slot];
+  lw[kLwTarget] = tol * tol * n;
+  lw[kLwLo] = -10.0;
+  lw[kLwStep] = 50.0 / (kLam - 1);
+  lw[kLwSkip] = 0.0;
+  lw[kLwErr] = 0.0;
+}
+
+// Coarse grid result: the first multiplier whose orders meet the target;
+// a bracket inside which the fine grid runs, or the final value at either
+// end of the grid (the fine sweep is skipped).
+__global__ void k_lam_coarse(const double* __restrict__ sums, double* __restrict__ lw) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double target = lw[kLwTarget], lo = lw[kLwLo], step = lw[kLwStep];
+  int k = 0;
+  while (k < kLam && sums[k] > target) ++k;
+  if (k == kLam || k == 0) {  // even the highest orders miss (best effort), or the lowest meet it
+    lw[kLwLogLam] = k == 0 ? lo : lo + (kLam - 1) * step;
+    lw[kLwSkip] = 1.0;
+  } else {
+    lw[kLwLo] = lo + (k - 1) * step;
+    lw[kLwStep] = step / (kLam - 1);
+  }
+}
+
+__global__ void k_lam_fine(const double* __restrict__ sums, double* __restrict__ lw) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || lw[kLwSkip] != 0.0) return;
+  const double target = lw[kLwTarget];
+  int j = 0;
+  while (j < kLam && sums[j] > target) ++j;
+  lw[kLwLogLam] = lw[kLwLo] + (j < kLam - 1 ? j : kLam - 1) * lw[kLwStep];
+}
+
+// lw[slot] = sqrt(lw[kLwErr] / n): a predicted norm-wise error
+__global__ void k_err_finish(double* __restrict__ lw, int slot, double n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) lw[slot] = sqrt(lw[kLwErr] / n);
+}
+
+__global__ void k_lw_zero(double* __restrict__ lw, int slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) lw[slot] = 0.0;
+}
+
+__global__ void k_lw_set(double* __restrict__ lw, int slot, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) lw[slot] = v;
+}
+
+// Whether to re-allocate for the slack the tile maxima leave: with e1 the
+// effective error of the first allocation, tol2 = min(tol^2 / e1, 4 tol)
+// when e1 < 0.9 tol.
+__global__ void k_realloc_decide(double* __restrict__ lw) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double tol = lw[kLwTol], e1 = lw[kLwEff1];
+  const bool re = e1 > 0.0 && e1 < 0.9 * tol;
+  lw[kLwRealloc] = re ? 1.0 : 0.0;
+  lw[kLwTol2] = re ? fmin(tol * tol / e1, 4.0 * tol) : tol;
+}
+
+// Keeps the second allocation (leaf, cell and tile orders) when it was made
+// and its effective error still meets the tolerance; lw[kLwEffFinal] = the
+// effective error of the kept one.
+__global__ void k_realloc_select(int nl, int nc, int nt, int* __restrict__ lq, const int* __restrict__ lq2,
+                                 int* __restrict__ cq, const int* __restrict__ cq2, signed char* __restrict__ to,
+                                 const signed char* __restrict__ to2, double* lw) {
+  const bool keep2 = lw[kLwRealloc] != 0.0 && lw[kLwEff2] <= lw[kLwTol];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) lw[kLwEffFinal] = keep2 ? lw[kLwEff2] : lw[kLwEff1];
+  if (!keep2) return;
+  if (i < nl) lq[i] = lq2[i];
+  if (i < nc) cq[i] = cq2[i];
+  if (i < nt) to[i] = to2[i];
+}
+
+// F_s from the judge's direct sums Fd [J][outsz]: the rms |F| (field
+// components fo .. outsz-1) without the largest max(1, J/1000) of them.
+// One block; the dropped samples are found by repeated arg-max, the rest
+// summed directly (no cancellation against a dominant sample).
+constexpr int kScaleDropMax = 16;
+__global__ __launch_bounds__(256) void k_field_scale(int J, int outsz, int fo, const double* __restrict__ Fd,
+                                                     double* __restrict__ out) {
+  __shared__ double vb[256];
+  __shared__ int vi[256];
+  __shared__ int drop[kScaleDropMax];
+  const int tid = threadIdx.x;
+  const int k = J > 0 ? min(max(1, J / 1000), kScaleDropMax) : 0;
+  auto m2 = [&](int i) {
+    double a = 0.0;
+    for (int c = fo; c < outsz; ++c) a += Fd[(size_t)i * outsz + c] * Fd[(size_t)i * outsz + c];
+    return a;
+  };
+  auto dropped = [&](int i, int r) {
+    for (int u = 0; u < r; ++u)
+      if (drop[u] == i) return true;
+    return false;
+  };
+  for (int r = 0; r < k; ++r) {
+    double best = -1.0;
+    int bi = -1;
+    for (int i = tid; i < J; i += 256) {
+      const double v = m2(i);
+      if (!dropped(i, r) && v > best) {
+        best = v;
+        bi = i;
+      }
+    }
+    vb[tid] = best;
+    vi[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s && (vb[tid + s] > vb[tid] || (vb[tid + s] == vb[tid] && vi[tid + s] < vi[tid] && vi[tid + s] >= 0))) {
+        vb[tid] = vb[tid + s];
+        vi[tid] = vi[tid + s];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) drop[r] = vi[0];
+    __syncthreads();
+  }
+  double sum = 0.0;
+  for (int i = tid; i < J; i += 256)
+    if (!dropped(i, k)) sum += m2(i);
+  vb[tid] = sum;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) vb[tid] += vb[tid + s];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const int keep = J - k;
+    *out = keep > 0 ? sqrt(vb[0] / keep) : 0.0;
+  }
+}
+
 }  // namespace
 
-double effective_error(Tree& t, const float* Y, double margin, int qmax, hipStream_t st) {
-  const int nc = t.ncells, nl = t.nleaves, nt = t.sched.ntiles;
-  if (nl == 0) return 0.0;
-  if (!t.sched.cell_tile_ready) {
-    t.sched.cell_tile.alloc(nc);
-    k_fill_i<<<nb(nc, 256), 256, 0, st>>>(nc, t.sched.cell_tile.get(), -1);
-    if (nt > 0)
-      k_cell_tile<<<nb((long long)nt * kTile, 256), 256, 0, st>>>(nt, t.sched.tile_cells.get(),
-                                                                 t.sched.cell_tile.get());
-    t.sched.cell_tile_ready = true;
+void field_scale_from_samples(Tree& t, int J, int outsz, int fo, hipStream_t st) {
+  learned_scalars(t);
+  k_field_scale<<<1, 256, 0, st>>>(J, outsz, fo, t.judge_Fd.get(), t.lw.get() + kLwScale);
+}
+
+void ensure_cell_tile(Tree& t, hipStream_t st) {
+  const int nc = t.ncells, nt = t.sched.ntiles;
+  if (t.sched.cell_tile_ready) return;
+  t.sched.cell_tile.alloc(nc);
+  k_fill_i<<<nb(nc, 256), 256, 0, st>>>(nc, t.sched.cell_tile.get(), -1);
+  if (nt > 0)
+    k_cell_tile<<<nb((long long)nt * t.sched.tile_size, 256), 256, 0, st>>>(nt, t.sched.tile_size,
+                                                                           t.sched.tile_cells.get(),
+                                                                           t.sched.cell_tile.get());
+  t.sched.cell_tile_ready = true;
+}
+
+void learned_scalars(Tree& t) {
+  if (t.lw.n < (size_t)kLwCount) {
+    t.lw.alloc(kLwCount);
+    FMM_HIP(hipMemset(t.lw.get(), 0, sizeof(double) * kLwCount));
   }
-  t.cell_eff.alloc(nc);
-  for (int l = 0; l < t.depth; ++l) {
-    const int b = t.level_off[l], e = t.level_off[l + 1];
-    k_eff_down<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.parent.get(), t.sched.cell_tile.get(), t.tile_order.get(),
-                                               qmax, t.cell_eff.get());
+}
+
+void effective_error(Tree& t, const float* Y, double margin, int qmax, const signed char* tile_order, int slot,
+                     hipStream_t st) {
+  const int nc = t.ncells, nl = t.nleaves;
+  k_lw_zero<<<1, 64, 0, st>>>(t.lw.get(), kLwErr);
+  if (nl > 0) {
+    ensure_cell_tile(t, st);
+    t.cell_eff.alloc(nc);
+    for (int l = 0; l < t.depth; ++l) {
+      const int b = t.level_off[l], e = t.level_off[l + 1];
+      k_eff_down<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.parent.get(), t.sched.cell_tile.get(), tile_order, qmax,
+                                                 t.cell_eff.get());
+    }
+    k_eff_err<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaves.get(), t.leaf_feat.get(), Y, t.cell_eff.get(),
+                                           (float)margin, t.lw.get() + kLwErr);
   }
-  DevBuf<double> acc;
-  acc.alloc(1);
-  k_zero_d<<<1, 64, 0, st>>>(1, acc.get());
-  k_eff_err<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaves.get(), t.leaf_feat.get(), Y, t.cell_eff.get(), (float)margin,
-                                         acc.get());
-  double h = 0.0;
-  FMM_HIP(hipMemcpyAsync(&h, acc.get(), sizeof(double), hipMemcpyDeviceToHost, st));
-  FMM_HIP(hipStreamSynchronize(st));
-  return sqrt(h / (double)t.n);
+  k_err_finish<<<1, 64, 0, st>>>(t.lw.get(), slot, (double)std::max<size_t>(t.n, 1));
 }
 
 double m2l_cell_macs(Tree& t, hipStream_t st) {
@@ -420,79 +576,67 @@ double m2l_cell_macs(Tree& t, hipStream_t st) {
   return h;
 }
 
-double allocate_orders(Tree& t, const float* Y, double tol, double margin, int qmin, int qmax, hipStream_t st) {
+void allocate_orders(Tree& t, const float* Y, int tol_slot, double margin, int qmin, int qmax, int* q_out,
+                     int pred_slot, bool gated, hipStream_t st) {
   const int nl = t.nleaves;
-  if (nl == 0) return 0.0;
-  DevBuf<double> acc;
-  acc.alloc(kLam);
-  const double target = tol * tol * (double)t.n;
+  const double n = (double)std::max<size_t>(t.n, 1);
+  t.lam_sums.alloc(kLam);
   // two passes of kLam multipliers (a coarse grid over 10^-10 .. 10^40, then
   // a fine grid inside the bracket): the smallest multiplier whose orders
-  // meet the target
-  auto sweep = [&](double lo, double step, std::vector<double>& out) {
-    k_zero_d<<<1, kLam, 0, st>>>(kLam, acc.get());
-    k_alloc_multi<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, lo, step, (float)margin, qmin, qmax,
-                                               acc.get());
-    out.resize(kLam);
-    FMM_HIP(hipMemcpyAsync(out.data(), acc.get(), sizeof(double) * kLam, hipMemcpyDeviceToHost, st));
-    FMM_HIP(hipStreamSynchronize(st));
-  };
-  std::vector<double> e;
-  double lo = -10.0, step = 50.0 / (kLam - 1);
-  sweep(lo, step, e);
-  int k = 0;
-  while (k < kLam && e[k] > target) ++k;
-  double loglam;
-  if (k == kLam) {
-    loglam = lo + (kLam - 1) * step;  // even the highest orders miss: best effort
-  } else if (k == 0) {
-    loglam = lo;
-  } else {
-    const double flo = lo + (k - 1) * step;
-    const double fstep = step / (kLam - 1);
-    sweep(flo, fstep, e);
-    int j = 0;
-    while (j < kLam && e[j] > target) ++j;
-    loglam = flo + std::min(j, kLam - 1) * fstep;
+  // meet the target; every decision on the device
+  k_lam_init<<<1, 64, 0, st>>>(t.lw.get(), tol_slot, n);
+  if (nl > 0) {
+    const int g = gated ? 1 : 0;
+    k_zero_d<<<1, kLam, 0, st>>>(kLam, t.lam_sums.get());
+    k_alloc_multi<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, t.lw.get(), (float)margin, qmin, qmax,
+                                               t.lam_sums.get(), g);
+    k_lam_coarse<<<1, 64, 0, st>>>(t.lam_sums.get(), t.lw.get());
+    k_zero_d<<<1, kLam, 0, st>>>(kLam, t.lam_sums.get());
+    k_alloc_multi<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, t.lw.get(), (float)margin, qmin, qmax,
+                                               t.lam_sums.get(), g);
+    k_lam_fine<<<1, 64, 0, st>>>(t.lam_sums.get(), t.lw.get());
+    // the orders of the chosen multiplier
+    k_alloc<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, t.lw.get(), (float)margin, qmin, qmax, q_out,
+                                         t.lw.get() + kLwErr, g);
   }
-  // the orders of the chosen multiplier
-  k_zero_d<<<1, 64, 0, st>>>(1, acc.get());
-  k_alloc<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_feat.get(), Y, loglam, (float)margin, qmin, qmax, t.leaf_q.get(),
-                                       acc.get());
-  double h = 0.0;
-  FMM_HIP(hipMemcpyAsync(&h, acc.get(), sizeof(double), hipMemcpyDeviceToHost, st));
-  FMM_HIP(hipStreamSynchronize(st));
-  return sqrt(h / (double)t.n);
+  k_err_finish<<<1, 64, 0, st>>>(t.lw.get(), pred_slot, n);
 }
 
-void chunk_classes(Tree& t, hipStream_t st, int* start) {
+void chunk_classes_async(Tree& t, hipStream_t st) {
   constexpr int NQ = M2LSchedule::kMaxOrder;
   M2LSchedule& S = t.sched;
   const int nch = S.nchunks;
-  for (int q = 0; q <= NQ; ++q) start[q] = nch;
+  S.class_start.alloc(NQ + 1);
+  k_fill_i<<<1, 64, 0, st>>>(NQ + 1, S.class_start.get(), -1);
   if (nch == 0) return;
-  DevBuf<unsigned long long> k0, k1;
-  DevBuf<int> st_d;
-  k0.alloc(nch);
-  k1.alloc(nch);
-  st_d.alloc(NQ + 1);
+  S.class_keys.alloc(2 * (size_t)nch);
+  unsigned long long* k0 = S.class_keys.get();
+  unsigned long long* k1 = k0 + nch;
   S.class_chunks.alloc(nch);
-  k_fill_i<<<1, 64, 0, st>>>(NQ + 1, st_d.get(), -1);
-  k_class_keys<<<nb(nch, 256), 256, 0, st>>>(nch, S.chunk_tile.get(), t.tile_order.get(), k0.get());
+  k_class_keys<<<nb(nch, 256), 256, 0, st>>>(nch, S.chunk_tile.get(), t.tile_order.get(), k0);
   size_t bytes = 0;
-  FMM_HIP(rocprim::radix_sort_keys(nullptr, bytes, k0.get(), k1.get(), (size_t)nch, 0, 40, st));
+  FMM_HIP(rocprim::radix_sort_keys(nullptr, bytes, k0, k1, (size_t)nch, 0, 40, st));
   void* tmp = t.scratch.get(bytes);
-  FMM_HIP(rocprim::radix_sort_keys(tmp, bytes, k0.get(), k1.get(), (size_t)nch, 0, 40, st));
-  k_class_split<<<nb(nch, 256), 256, 0, st>>>(nch, k1.get(), S.class_chunks.get(), st_d.get());
-  std::vector<int> h(NQ + 1);
-  FMM_HIP(hipMemcpyAsync(h.data(), st_d.get(), sizeof(int) * (NQ + 1), hipMemcpyDeviceToHost, st));
+  FMM_HIP(rocprim::radix_sort_keys(tmp, bytes, k0, k1, (size_t)nch, 0, 40, st));
+  k_class_split<<<nb(nch, 256), 256, 0, st>>>(nch, k1, S.class_chunks.get(), S.class_start.get());
+}
+
+int read_orders(Tree& t, hipStream_t st, int* start) {
+  constexpr int NQ = M2LSchedule::kMaxOrder;
+  const int nch = t.sched.nchunks;
+  std::vector<int> h(NQ + 2, -1);
+  FMM_HIP(hipMemcpyAsync(h.data(), t.cell_q.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+  if (nch > 0)
+    FMM_HIP(hipMemcpyAsync(h.data() + 1, t.sched.class_start.get(), sizeof(int) * (NQ + 1), hipMemcpyDeviceToHost,
+                           st));
   FMM_HIP(hipStreamSynchronize(st));
   // empty classes start where the next non-empty one does
   int next = nch;
   for (int q = NQ; q >= 0; --q) {
-    if (h[q] >= 0) next = h[q];
+    if (h[1 + q] >= 0) next = h[1 + q];
     start[q] = next;
   }
+  return h[0];
 }
 
 void leaf_features(Tree& t, int ss, hipStream_t st) {
@@ -501,6 +645,7 @@ void leaf_features(Tree& t, int ss, hipStream_t st) {
   t.cell_q2.alloc(nc);
   t.cell_acc.alloc((size_t)nc * kAcc);
   t.leaf_feat.alloc((size_t)std::max(nl, 1) * kMlpF);
+  learned_scalars(t);
   k_zero_d<<<nb(nc, 256), 256, 0, st>>>(nc, t.cell_qabs.get());
   k_zero_d<<<nb(nc, 256), 256, 0, st>>>(nc, t.cell_q2.get());
   if (nl == 0) return;
@@ -517,8 +662,8 @@ void leaf_features(Tree& t, int ss, hipStream_t st) {
     const int b = t.level_off[l], e = t.level_off[l + 1];
     k_acc_down<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.parent.get(), t.cell_acc.get());
   }
-  k_leaf_feat<<<nb(nl, 128), 128, 0, st>>>(nl, t.leaves.get(), t.pcount.get(), t.cell_acc.get(), t.judge_scale,
-                                          t.leaf_feat.get());
+  k_leaf_feat<<<nb(nl, 128), 128, 0, st>>>(nl, t.leaves.get(), t.pcount.get(), t.cell_acc.get(),
+                                          t.lw.get() + kLwScale, t.leaf_feat.get());
 }
 
 void leaf_error(Tree& t, const double* F, const double* Fref, int outsz, int fo, double* err, hipStream_t st) {
@@ -528,23 +673,42 @@ void leaf_error(Tree& t, const double* F, const double* Fref, int outsz, int fo,
                                          fo, F, Fref, err);
 }
 
-int orders_from_leaves(Tree& t, hipStream_t st) {
-  const int nc = t.ncells, nl = t.nleaves;
-  t.cell_q.alloc(nc);
-  k_fill_i<<<nb(nc, 256), 256, 0, st>>>(nc, t.cell_q.get(), 0);
-  k_leaf_to_cell<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaves.get(), t.leaf_q.get(), t.cell_q.get());
+void orders_from_leaves(Tree& t, const int* leaf_q, int* cell_q, signed char* tile_order, hipStream_t st) {
+  const int nc = t.ncells, nl = t.nleaves, nt = t.sched.ntiles;
+  k_fill_i<<<nb(nc, 256), 256, 0, st>>>(nc, cell_q, 0);
+  if (nl > 0) k_leaf_to_cell<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaves.get(), leaf_q, cell_q);
   for (int l = t.depth - 2; l >= 0; --l) {
     const int b = t.level_off[l], e = t.level_off[l + 1];
-    k_parent_max<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cell_q.get());
+    k_parent_max<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), cell_q);
   }
-  const int nt = t.sched.ntiles;
-  t.tile_order.alloc(std::max(nt, 1));
   if (nt > 0)
-    k_tile_order<<<nb(nt, 256), 256, 0, st>>>(nt, t.sched.tile_cells.get(), t.cell_q.get(), t.tile_order.get());
-  int qmax = 0;
-  FMM_HIP(hipMemcpyAsync(&qmax, t.cell_q.get(), sizeof(int), hipMemcpyDeviceToHost, st));
-  FMM_HIP(hipStreamSynchronize(st));
-  return qmax;
+    k_tile_order<<<nb(nt, 256), 256, 0, st>>>(nt, t.sched.tile_size, t.sched.tile_cells.get(), cell_q, tile_order);
+}
+
+void select_orders(Tree& t, const float* Y, double margin, int qmin, int qmax, bool refine, hipStream_t st) {
+  const int nc = t.ncells, nl = t.nleaves, nt = t.sched.ntiles;
+  t.leaf_q.alloc(std::max(nl, 1));
+  t.leaf_q2.alloc(std::max(nl, 1));
+  t.cell_q.alloc(nc);
+  t.cell_q2a.alloc(nc);
+  t.tile_order.alloc(std::max(nt, 1));
+  t.tile_order2.alloc(std::max(nt, 1));
+  k_lw_set<<<1, 64, 0, st>>>(t.lw.get(), kLwTol, t.tol);
+  allocate_orders(t, Y, kLwTol, margin, qmin, qmax, t.leaf_q.get(), kLwPred, false, st);
+  orders_from_leaves(t, t.leaf_q.get(), t.cell_q.get(), t.tile_order.get(), st);
+  if (!refine) return;
+  // the tile maxima leave the predicted error below the budget: allocate
+  // once more for the tolerance scaled by that slack, keep it when its
+  // effective error still meets the tolerance
+  effective_error(t, Y, margin, qmax, t.tile_order.get(), kLwEff1, st);
+  k_realloc_decide<<<1, 64, 0, st>>>(t.lw.get());
+  allocate_orders(t, Y, kLwTol2, margin, qmin, qmax, t.leaf_q2.get(), kLwPred2, true, st);
+  orders_from_leaves(t, t.leaf_q2.get(), t.cell_q2a.get(), t.tile_order2.get(), st);
+  effective_error(t, Y, margin, qmax, t.tile_order2.get(), kLwEff2, st);
+  const int m = std::max(std::max(nl, nc), nt);
+  k_realloc_select<<<nb(m, 256), 256, 0, st>>>(nl, nc, nt, t.leaf_q.get(), t.leaf_q2.get(), t.cell_q.get(),
+                                               t.cell_q2a.get(), t.tile_order.get(), t.tile_order2.get(),
+                                               t.lw.get());
 }
 
 void raise_failing_leaves(Tree& t, int qfull, hipStream_t st) {
@@ -565,7 +729,7 @@ double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out) {
   DevBuf<double> acc;
   acc.alloc(2);
   k_zero_d<<<1, 64, 0, st>>>(2, acc.get());
-  if (!t.sched.tile_pairs_ready) {  // order independent: once per tree
+  if (!t.sched.tile_pairs_ready) {  // order independent: once per tile set
     t.sched.tile_pairs.alloc(nt);
     k_tile_pairs<<<nt, 256, 0, st>>>(nt, t.sched.tile_group_off.get(), t.sched.group_src.get(),
                                      t.sched.tile_pairs.get());

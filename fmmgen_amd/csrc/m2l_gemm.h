@@ -52,13 +52,15 @@ __host__ __device__ constexpr int slot_degree(int j) {
   return d;
 }
 
-template <int P, int SO>
-struct M2LPlan {
+// MFMA steps of one translation: row blocks of 16 L slots by slices of KW
+// M slots, the steps whose lowest degrees sum to more than P dropped.
+template <int P, int SO, int KW>
+struct StepPlan {
   static constexpr int LS = lsize(P, SO), MS = msize(P, SO);
-  static constexpr int NI = (LS + 15) / 16, NK = (MS + 3) / 4;
+  static constexpr int NI = (LS + 15) / 16, NK = (MS + KW - 1) / KW;
   static constexpr int OFF = nterms(SO - 1);  // M slot -> full index
   static constexpr bool valid(int I, int K) {
-    return 16 * I < LS && 4 * K < MS && slot_degree(16 * I) + slot_degree(4 * K + OFF) <= P;
+    return 16 * I < LS && KW * K < MS && slot_degree(16 * I) + slot_degree(KW * K + OFF) <= P;
   }
   static constexpr int count_steps() {
     int s = 0;
@@ -95,6 +97,10 @@ struct M2LPlan {
   static constexpr Tab tab = make();
   static constexpr int NSTEP = tab.base[NK];
 };
+template <int P, int SO>
+using M2LPlan = StepPlan<P, SO, 4>;    // FP32 / FP64 16x16x4 MFMAs
+template <int P, int SO>
+using M2LPlan32 = StepPlan<P, SO, 32>;  // BF16 16x16x32 MFMAs
 
 // Multi-index of slot j of the graded layout and its inverse:
 // idx(n) = Nterms(|n| - 1) + T(|n| - n_x) + (|n| - n_x - n_y), T(a) = a(a+1)/2.
@@ -181,9 +187,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 // row cut between waves is added into L by each).  Within the wave steps
 // are ordered by slice K, so consecutive steps of one slice share their B
 // operand read.
-template <int P, int SO, int W, int WV>
+template <class Plan, int W, int WV>
 struct WavePlan {
-  using Plan = M2LPlan<P, SO>;
   struct Tab {
     short s[Plan::NSTEP + 1];
     int n, nrow;
@@ -253,10 +258,9 @@ struct WavePlan {
 // Round-robin plan: wave wv takes the row blocks I = wv mod W, its steps
 // in slice-major order (consecutive steps of one slice share the B read).
 // Used below order 6, where a block's few steps leave little to balance.
-template <int P, int SO, int W, int WV>
+template <class Plan, int W, int WV>
 struct WavePlanRR {
-  using Plan = M2LPlan<P, SO>;
-  using Tab = typename WavePlan<P, SO, W, WV>::Tab;
+  using Tab = typename WavePlan<Plan, W, WV>::Tab;
   static constexpr Tab make() {
     Tab t{};
     int n = 0;
@@ -278,7 +282,21 @@ struct WavePlanRR {
 // The plan a kernel of order P uses (measured: the balanced plan is faster
 // from order 7 up, equal at 6).
 template <int P, int SO, int W, int WV>
-using WavePlanFor = std::conditional_t<(P >= 6), WavePlan<P, SO, W, WV>, WavePlanRR<P, SO, W, WV>>;
+using WavePlanFor =
+    std::conditional_t<(P >= 6), WavePlan<M2LPlan<P, SO>, W, WV>, WavePlanRR<M2LPlan<P, SO>, W, WV>>;
+template <int P, int SO, int W, int WV>
+using WavePlan32For = WavePlan<M2LPlan32<P, SO>, W, WV>;
+
+// Steps before wave wv in the BF16 plan's index table (4 words per lane
+// and step: eight 16-bit D slots).
+template <int P, int SO, int W, int WV>
+constexpr int wave_steps32() {
+  if constexpr (WV == 0) {
+    return 0;
+  } else {
+    return wave_steps32<P, SO, W, WV - 1>() + WavePlan32For<P, SO, W, WV - 1>::tab.n;
+  }
+}
 
 // The index table holds, per wave, its steps' 8-bit D offsets packed four
 // to a 32-bit word (one LDS read serves four steps): word d of lane l of
@@ -464,46 +482,14 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 // matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  Operand
 // layout (measured on gfx950): A[l%16][8 (l/16) + i], B[8 (l/16) + j][l%16],
 // i, j = 0..7; the accumulator is laid out as for the f32 16x16 MFMA.
-template <int P, int SO>
-struct M2LPlan32 {
-  static constexpr int LS = lsize(P, SO), MS = msize(P, SO);
-  static constexpr int NI = (LS + 15) / 16, NK = (MS + 31) / 32;
-  static constexpr int OFF = nterms(SO - 1);
-  static constexpr bool valid(int I, int K) {
-    return 16 * I < LS && 32 * K < MS && slot_degree(16 * I) + slot_degree(32 * K + OFF) <= P;
-  }
-  static constexpr int count_steps() {
-    int s = 0;
-    for (int K = 0; K < NK; ++K)
-      for (int I = 0; I < NI; ++I)
-        if (valid(I, K)) s = s + 1;
-    return s;
-  }
-  static constexpr int NST0 = count_steps();
-  struct Tab {
-    int nI[NK];
-    int base[NK + 1];
-  };
-  static constexpr Tab make() {
-    Tab t{};
-    int s = 0;
-    for (int K = 0; K < NK; ++K) {
-      int c = 0;
-      for (int I = 0; I < NI; ++I)
-        if (valid(I, K)) c = I + 1;
-      t.nI[K] = c;
-      t.base[K] = s;
-      s += c;
-    }
-    t.base[NK] = s;
-    return t;
-  }
-  static constexpr Tab tab = make();
-  static constexpr int NSTEP = tab.base[NK];
-};
-
+//
+// Organised as k_m2l_gemm: W waves per block share a tile chunk's staged
+// operands (the group's D vector split once into packed hi | lo, the 16
+// source columns in FP32) and deal the steps of the 16x16x32 plan by the
+// balanced wave plan.  Each lane's eight D slots of each of its steps are
+// held in registers (four words of 16-bit slots, loaded once per block),
+// so a step costs eight D reads from LDS and no index reads.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // One FP32 value as packed (hi | lo << 16) bf16 halves.
@@ -534,147 +520,188 @@ __device__ __forceinline__ void split_bf16(const float (&v)[8], bf16x8& hi, bf16
   }
 }
 
+// Waves per block of the BF16 kernel: a wave's index words (4 per step)
+// stay within about 40 registers.
 template <int P, int SO>
-__global__ __launch_bounds__(64) void k_m2l_bf16x3(int nchunks, int nblocks, const int* __restrict__ tile_cells,
-                                                   const int* __restrict__ chunk_tile,
-                                                   const int* __restrict__ chunk_g0,
-                                                   const int* __restrict__ chunk_g1,
-                                                   const int* __restrict__ group_key,
-                                                   const int* __restrict__ group_src,
-                                                   const int* __restrict__ key_dlevel,
-                                                   const float* __restrict__ Dtab, const short* __restrict__ idxtab,
-                                                   const signed char* __restrict__ tile_order,
-                                                   const int* __restrict__ chunk_list, int mstride, int lstride,
-                                                   const float* __restrict__ M, float* __restrict__ L) {
+constexpr int m2l_waves32() {
+  return M2LPlan32<P, SO>::NSTEP > 40 ? 8 : 4;
+}
+
+template <int P, int SO, int W>
+__global__ __launch_bounds__(64 * W) void k_m2l_bf16x3(int nchunks, int nblocks, const int* __restrict__ tile_cells,
+                                                       const int* __restrict__ chunk_tile,
+                                                       const int* __restrict__ chunk_g0,
+                                                       const int* __restrict__ chunk_g1,
+                                                       const int* __restrict__ group_key,
+                                                       const int* __restrict__ group_src,
+                                                       const int* __restrict__ key_dlevel,
+                                                       const float* __restrict__ Dtab,
+                                                       const unsigned* __restrict__ idxtab,
+                                                       const signed char* __restrict__ tile_order,
+                                                       const int* __restrict__ chunk_list, int mstride, int lstride,
+                                                       const float* __restrict__ M, float* __restrict__ L) {
   using Plan = M2LPlan32<P, SO>;
-  constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NK = Plan::NK, NST = Plan::NSTEP;
+  constexpr int MS = Plan::MS, LS = Plan::LS, NK = Plan::NK;
   constexpr int RS = ((32 * NK + 3) / 4) * 4 + 4;  // row stride: 16-byte aligned reads, zero tail
   constexpr int NJ = (MS + 63) / 64;
+  constexpr int CPW = kTile / W;
+  constexpr int DJ = (NJ + W - 1) / W;
   using V4 = typename Vec4<float>::type;
-  __shared__ short sidx[NST * 8 * 64];
-  __shared__ unsigned sD[MS + 1];  // D split once per group: packed (hi | lo << 16)
-  __shared__ float sB[kTile * RS];
+  __shared__ unsigned sD[MS + 1];  // D split once per group: packed (hi | lo << 16); slot MS stays zero
+  __shared__ __attribute__((aligned(16))) float sB[kTile * RS];
   __shared__ signed char sdeg[MS];
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int ci = xcd_tile(blockIdx.x, nblocks);
   if (ci >= nchunks) return;
   const int chunk = chunk_list ? chunk_list[ci] : ci;
   const int tile = chunk_tile[chunk];
-  for (int i = lane; i < NST * 8 * 64; i += 64) {
-    const short v = idxtab[i];
-    sidx[i] = v < 0 ? (short)MS : v;
-  }
-  if (lane == 0) sD[MS] = 0u;
-  for (int i = lane; i < kTile * RS; i += 64) sB[i] = 0.f;
-  for (int j = lane; j < MS; j += 64) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
-  V4 acc[NI];
-#pragma unroll
-  for (int I = 0; I < NI; ++I) acc[I] = V4{0, 0, 0, 0};
+  if (tid == 0) sD[MS] = 0u;
+  for (int i = tid; i < kTile * RS; i += 64 * W) sB[i] = 0.f;
+  for (int j = tid; j < MS; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
   const int q = tile_order ? (int)tile_order[tile] : P;
   const int qc = q < 0 ? 0 : (q > P ? P : q);
-  float rD[NJ], rB[kTile][NJ];
-  auto prefetch = [&](int g) {
-    const int key = group_key[g];
+
+  dispatch_wave<W>(w, [&](auto WV) {
+    constexpr int wv = WV;
+    using WP = WavePlan32For<P, SO, W, wv>;
+    constexpr int NIW = WP::tab.nrow;
+    constexpr int NSW = WP::tab.n;
+    constexpr int WS = wave_steps32<P, SO, W, wv>();
+    u32x4 ix[NSW > 0 ? NSW : 1];
+    static_for<NSW>([&](auto i) {
+      ix[i] = *reinterpret_cast<const u32x4*>(idxtab + ((size_t)(WS + i) * 64 + lane) * 4);
+    });
+    V4 acc[NIW > 0 ? NIW : 1];
 #pragma unroll
-    for (int i = 0; i < NJ; ++i) {
-      const int j = 64 * i + lane;
-      rD[i] = j < MS ? Dtab[(size_t)key * mstride + j] : 0.f;
-    }
+    for (int i = 0; i < (NIW > 0 ? NIW : 1); ++i) acc[i] = V4{0, 0, 0, 0};
+    float rD[DJ], rB[CPW][NJ];
+    // loads are issued unconditionally and masked afterwards
+    auto prefetch = [&](int g) {
+      const int key = group_key[g];
 #pragma unroll
-    for (int c = 0; c < kTile; ++c) {
-      const int src = group_src[(size_t)g * kTile + c];
-#pragma unroll
-      for (int i = 0; i < NJ; ++i) {
-        const int j = 64 * i + lane;
-        rB[c][i] = (src >= 0 && j < MS) ? M[(size_t)src * mstride + j] : 0.f;
+      for (int i = 0; i < DJ; ++i) {
+        const int j = 64 * (wv + W * i) + lane;
+        const float v = Dtab[(size_t)key * mstride + (j < MS ? j : MS - 1)];
+        rD[i] = j < MS ? v : 0.f;
       }
-    }
-  };
-  if (g0 < g1) prefetch(g0);
-  for (int g = g0; g < g1; ++g) {
-    const int dl = key_dlevel[group_key[g]];
-    __syncthreads();
 #pragma unroll
-    for (int i = 0; i < NJ; ++i) {
-      const int j = 64 * i + lane;
-      if (j < MS) sD[j] = sdeg[j] <= qc ? pack_split(rD[i]) : 0u;
-    }
+      for (int c = 0; c < CPW; ++c) {
+        const int src = group_src[(size_t)g * kTile + wv + W * c];
+        const float* row = M + (size_t)(src >= 0 ? src : 0) * mstride;
 #pragma unroll
-    for (int c = 0; c < kTile; ++c)
-#pragma unroll
-      for (int i = 0; i < NJ; ++i) {
-        const int j = 64 * i + lane;
-        if (j < MS) sB[c * RS + j] = dl != 0 ? ldexpf(rB[c][i], dl * sdeg[j]) : rB[c][i];
+        for (int i = 0; i < NJ; ++i) {
+          const int j = 64 * i + lane;
+          const float v = row[j < MS ? j : MS - 1];
+          rB[c][i] = (src >= 0 && j < MS) ? v : 0.f;
+        }
       }
-    __syncthreads();
-    if (g + 1 < g1) prefetch(g + 1);
-    static_for<NK>([&](auto K) {
-      constexpr int nI = Plan::tab.nI[K];
-      constexpr int dK = slot_degree(32 * K + Plan::OFF);
-      if (dK > qc) return;  // slices past the tile's order
-      float bv[8];
+    };
+    if (g0 < g1) prefetch(g0);
+    for (int g = g0; g < g1; ++g) {
+      const int dl = key_dlevel[group_key[g]];
+      __syncthreads();  // previous group finished reading sD / sB
 #pragma unroll
-      for (int j = 0; j < 8; ++j) bv[j] = sB[col * RS + 32 * K + 8 * kk + j];
-      bf16x8 bhi, blo;
-      split_bf16(bv, bhi, blo);
-      static_for<nI>([&](auto I) {
-        constexpr int s = Plan::tab.base[K] + I;
-        if (slot_degree(16 * I) + dK > qc) return;
+      for (int i = 0; i < DJ; ++i) {
+        const int j = 64 * (wv + W * i) + lane;
+        if (j < MS) sD[j] = sdeg[j] <= qc ? pack_split(rD[i]) : 0u;
+      }
+#pragma unroll
+      for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int i = 0; i < NJ; ++i) {
+          const int j = 64 * i + lane;
+          if (j < MS) sB[(wv + W * c) * RS + j] = dl != 0 ? ldexpf(rB[c][i], dl * sdeg[j]) : rB[c][i];
+        }
+      __syncthreads();  // staged operands visible to every wave
+      if (g + 1 < g1) prefetch(g + 1);  // in flight during the MFMAs below
+      bf16x8 bH, bL;
+      static_for<NSW>([&](auto i) {
+        constexpr int s = WP::tab.s[i];
+        constexpr int K = Plan::tab.sK[s], I = Plan::tab.sI[s];
+        constexpr int KP = i > 0 ? Plan::tab.sK[WP::tab.s[i > 0 ? i - 1 : 0]] : -1;
+        if constexpr (K != KP) {  // consecutive steps of one slice share the B operand
+          const float* src = sB + col * RS + 32 * K + 8 * kk;
+          const float4 v0 = *reinterpret_cast<const float4*>(src);
+          const float4 v1 = *reinterpret_cast<const float4*>(src + 4);
+          const float bv[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+          split_bf16(bv, bH, bL);
+        }
         unsigned aw[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) aw[i] = sD[sidx[(s * 8 + i) * 64 + lane]];
-        bf16x8 ahi, alo;
-        unpack8(aw, ahi, alo);
-        acc[I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[I], 0, 0, 0);
-        acc[I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[I], 0, 0, 0);
-        acc[I] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[I], 0, 0, 0);
+        for (int e = 0; e < 4; ++e) {
+          aw[2 * e] = sD[ix[i][e] & 0xffffu];
+          aw[2 * e + 1] = sD[ix[i][e] >> 16];
+        }
+        bf16x8 aH, aL;
+        unpack8(aw, aH, aL);
+        constexpr int A = WP::tab.slot[I];
+        acc[A] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH, bH, acc[A], 0, 0, 0);
+        acc[A] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH, bL, acc[A], 0, 0, 0);
+        acc[A] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aL, bH, acc[A], 0, 0, 0);
       });
-    });
-  }
-  const int cell = tile_cells[tile * kTile + col];
-  if (cell < 0) return;
-  float* Lc = L + (size_t)cell * lstride;
-  static_for<NI>([&](auto I) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = 16 * I + acc_row<float>(kk, r);
-      if (n < LS) atomicAdd(&Lc[n], acc[I][r]);
     }
+    const int cell = tile_cells[tile * kTile + col];
+    if (cell < 0) return;
+    float* Lc = L + (size_t)cell * lstride;
+    static_for<NIW>([&](auto ii) {
+      constexpr int I = WP::tab.rows[ii];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = 16 * I + acc_row<float>(kk, r);
+        if (n < LS) atomicAdd(&Lc[n], acc[ii][r]);  // chunks of one tile add into the same L
+      }
+    });
   });
 }
 
-// (step, element, lane) -> D slot table of the BF16 plan (-1: zero entry).
-template <int P, int SO>
-std::vector<short> m2l_index_table32() {
-  using Plan = M2LPlan32<P, SO>;
-  std::vector<int> mx, my, mz;
-  for (int d = 0; d <= 2 * P; ++d)
-    for (int x = d; x >= 0; --x)
-      for (int y = d - x; y >= 0; --y) {
-        mx.push_back(x);
-        my.push_back(y);
-        mz.push_back(d - x - y);
-      }
-  auto idx = [](int x, int y, int z) {
-    const int d = x + y + z;
-    const int a = d - x;
-    return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
-  };
-  std::vector<short> t((size_t)Plan::NSTEP * 8 * 64, -1);
-  for (int K = 0; K < Plan::NK; ++K)
-    for (int I = 0; I < Plan::tab.nI[K]; ++I) {
-      const int s = Plan::tab.base[K] + I;
+// Host side: the BF16 plan's table for a W-wave block, per wave step (wave
+// order) and lane four words holding the eight 16-bit D slots
+// A[l%16][8 (l/16) + i] reads (MS, a zero slot, for an empty entry).
+template <int P, int SO, int W, int WV>
+void fill_wave_index32(std::vector<unsigned>& t) {
+  if constexpr (WV < W) {
+    using WP = WavePlan32For<P, SO, W, WV>;
+    using Plan = M2LPlan32<P, SO>;
+    std::vector<int> mx, my, mz;
+    for (int d = 0; d <= 2 * P; ++d)
+      for (int x = d; x >= 0; --x)
+        for (int y = d - x; y >= 0; --y) {
+          mx.push_back(x);
+          my.push_back(y);
+          mz.push_back(d - x - y);
+        }
+    auto idx = [](int x, int y, int z) {
+      const int d = x + y + z;
+      const int a = d - x;
+      return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
+    };
+    const size_t s0 = (size_t)wave_steps32<P, SO, W, WV>();
+    for (int i = 0; i < WP::tab.n; ++i) {
+      const int st = WP::tab.s[i];
+      const int I = Plan::tab.sI[st], K = Plan::tab.sK[st];
       for (int l = 0; l < 64; ++l)
-        for (int i = 0; i < 8; ++i) {
-          const int n = 16 * I + (l & 15), m = 32 * K + 8 * (l >> 4) + i;
-          if (n >= Plan::LS || m >= Plan::MS) continue;
-          const int mf = m + Plan::OFF;
-          if (mx[n] + my[n] + mz[n] + mx[mf] + my[mf] + mz[mf] > P) continue;
-          t[((size_t)s * 8 + i) * 64 + l] = (short)(idx(mx[n] + mx[mf], my[n] + my[mf], mz[n] + mz[mf]) - Plan::OFF);
+        for (int e = 0; e < 8; ++e) {
+          const int n = 16 * I + (l & 15), m = 32 * K + 8 * (l >> 4) + e;
+          unsigned u = (unsigned)Plan::MS;
+          if (n < Plan::LS && m < Plan::MS) {
+            const int mf = m + Plan::OFF;
+            if (mx[n] + my[n] + mz[n] + mx[mf] + my[mf] + mz[mf] <= P)
+              u = (unsigned)(idx(mx[n] + mx[mf], my[n] + my[mf], mz[n] + mz[mf]) - Plan::OFF);
+          }
+          unsigned& word = t[((s0 + i) * 64 + l) * 4 + e / 2];
+          word = (e % 2 == 0) ? ((word & 0xffff0000u) | u) : ((word & 0xffffu) | (u << 16));
         }
     }
+    fill_wave_index32<P, SO, W, WV + 1>(t);
+  }
+}
+
+template <int P, int SO, int W>
+std::vector<unsigned> m2l_index_table32() {
+  std::vector<unsigned> t((size_t)M2LPlan32<P, SO>::NSTEP * 64 * 4, 0u);
+  fill_wave_index32<P, SO, W, 0>(t);
   return t;
 }
 

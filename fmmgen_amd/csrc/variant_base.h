@@ -168,6 +168,10 @@ __global__ void k_judge_check(int J, const int* __restrict__ slots, const int* _
   }
 }
 
+__global__ void k_set_scalar(double* __restrict__ p, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 __global__ void k_fill_int(long long n, int* __restrict__ a) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0;
@@ -257,24 +261,20 @@ __device__ __forceinline__ void p2p_accumulate(const T (&part)[OUT], T (&sum)[OU
   }
 }
 
+// One work item of the near field (the body of k_p2p_leaf).
 template <typename T>
-__global__ __launch_bounds__(64) void k_p2p_leaf(int nitems, int nblocks, const int* __restrict__ it_leaf,
-                                                 const int* __restrict__ it_r, const int* __restrict__ it_o,
-                                                 const int* __restrict__ it_n, const int* __restrict__ pstart,
-                                                 const int* __restrict__ pcount, const int* __restrict__ rng_off,
-                                                 const int* __restrict__ rng_start, const int* __restrict__ rng_len,
-                                                 const double* __restrict__ px, const double* __restrict__ py,
-                                                 const double* __restrict__ pz, const double* __restrict__ pS,
-                                                 int split, double* __restrict__ F) {
+__device__ __forceinline__ void p2p_item(int it, int nitems, const int* __restrict__ it_leaf,
+                                         const int* __restrict__ it_r, const int* __restrict__ it_o,
+                                         const int* __restrict__ it_n, const int* __restrict__ pstart,
+                                         const int* __restrict__ pcount, const int* __restrict__ rng_off,
+                                         const int* __restrict__ rng_start, const int* __restrict__ rng_len,
+                                         const double* __restrict__ px, const double* __restrict__ py,
+                                         const double* __restrict__ pz, const double* __restrict__ pS, int split,
+                                         double* __restrict__ F, double* sx, double* sy, double* sz, T* sS,
+                                         int* sid) {
   constexpr bool kKahan = sizeof(T) == 4;
-  __shared__ double sx[64], sy[64], sz[64];
-  __shared__ T sS[64 * SS];
-  __shared__ int sid[64];
-  const int b = blockIdx.x;
-  const int it = (b % 8) * ((nblocks + 7) / 8) + b / 8;  // XCD-contiguous item ranges
-  if (it >= nitems) return;
-  const int A = it_leaf[it];
   const int lane = threadIdx.x;
+  const int A = it_leaf[it];
   const int t0 = pstart[A], nt = pcount[A];
   const int rb = rng_off[A], re = rng_off[A + 1];
   // a leaf cut into several items accumulates atomically
@@ -369,6 +369,40 @@ __global__ __launch_bounds__(64) void k_p2p_leaf(int nitems, int nblocks, const 
   }
 }
 
+// Persistent form: a grid of a few waves per SIMD takes work items
+// from eight counters, one per XCD range of items (blocks are dealt to the
+// XCDs round robin, and neighbouring leaves share their sources in that
+// XCD's L2); a block whose range is exhausted takes items of the next
+// XCDs'.  next[8] is zero before the launch.
+template <typename T>
+__global__ __launch_bounds__(64) void k_p2p_leaf(int nitems, int* __restrict__ next, const int* __restrict__ it_leaf,
+                                                 const int* __restrict__ it_r, const int* __restrict__ it_o,
+                                                 const int* __restrict__ it_n, const int* __restrict__ pstart,
+                                                 const int* __restrict__ pcount, const int* __restrict__ rng_off,
+                                                 const int* __restrict__ rng_start, const int* __restrict__ rng_len,
+                                                 const double* __restrict__ px, const double* __restrict__ py,
+                                                 const double* __restrict__ pz, const double* __restrict__ pS,
+                                                 int split, double* __restrict__ F) {
+  constexpr bool kKahan = sizeof(T) == 4;
+  __shared__ double sx[64], sy[64], sz[64];
+  __shared__ T sS[64 * SS];
+  __shared__ int sid[64];
+  const int lane = threadIdx.x;
+  const int per = (nitems + 7) / 8;
+  for (int k = 0; k < 8; ++k) {
+    const int x = (blockIdx.x + k) & 7;
+    const int lo = x * per, hi = min(nitems, lo + per);
+    for (;;) {
+      int got = 0;
+      if (lane == 0) got = atomicAdd(&next[x], 1);
+      const int it = lo + __shfl(got, 0, 64);
+      if (it >= hi) break;
+      p2p_item<T>(it, nitems, it_leaf, it_r, it_o, it_n, pstart, pcount, rng_off, rng_start, rng_len, px, py, pz,
+                  pS, split, F, sx, sy, sz, sS, sid);
+    }
+  }
+}
+
 __global__ void k_fill_zero(long long n, double* __restrict__ a) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0;
@@ -382,9 +416,11 @@ void near_field(Tree& t, int mode, hipStream_t st) {
     k_fill_zero<<<(int)((m + 255) / 256), 256, 0, st>>>(m, t.Fp2p.get());
   }
   if (ni == 0) return;
-  const int grid = ((ni + 7) / 8) * 8;
+  t.p2p_next.alloc(8);
+  k_fill_int<<<1, 64, 0, st>>>(8, t.p2p_next.get());
+  const int grid = std::min(ni, p2p_grid(t));
   auto launch = [&](auto kern) {
-    kern<<<grid, 64, 0, st>>>(ni, grid, t.p2p_item_leaf.get(), t.p2p_item_r.get(), t.p2p_item_o.get(),
+    kern<<<grid, 64, 0, st>>>(ni, t.p2p_next.get(), t.p2p_item_leaf.get(), t.p2p_item_r.get(), t.p2p_item_o.get(),
                               t.p2p_item_n.get(), t.pstart.get(), t.pcount.get(), t.rng_off.get(),
                               t.rng_start.get(), t.rng_len.get(), t.px.get(), t.py.get(), t.pz.get(), t.pS.get(),
                               t.p2p_split, t.Fp2p.get());
@@ -444,25 +480,22 @@ void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
   OrderTable<PMIN>::far(t.order, t, mode, mask, st);
   if (mask & kPhaseP2P) near_field(t, mode, st);
   OrderTable<PMIN>::l2p(t.order, t, mode, mask, F, st);
+  t.ml_order = mode == 0 ? t.order : -1;  // FP64 M / L hold this evaluation's expansions
 }
 
-// Learned-order evaluation: the judge sample's FP64 direct sums and field
-// scale, leaf features, FP16 network inference -> leaf orders (the
-// cheapest whose predicted norm-wise error meets the tolerance) -> cell
-// and tile orders, FP32 P2P, mixed-precision far field at the largest
-// order with every M2L tile truncated at its own order, then the judge's
-// check at its kJudgeTargets targets.  When the sample's relative field error exceeds the tolerance,
-// the leaves holding a target whose error exceeds tol * rms |F| rerun at
-// the tree's order and the other leaves are re-allocated for half the
-// tolerance (all leaves run at the tree's order when more than one target
-// in kJudgeDistrust fails).
-// mode 3: FP32 M2L, mode 4: BF16x3 M2L.
-// M2L of the learned evaluation: chunks grouped by their tile's order,
-// each group launched with the kernel of its own order on the expansions
-// of the far field's order P (row strides of order P).
+// The caller's current source strengths in sorted order (t.pS).
+void gather_sources(Tree& t, hipStream_t st) {
+  const int n = (int)t.n;
+  t.pS.alloc((size_t)std::max(n, 1) * SS);
+  k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+}
+
+// M2L of the learned evaluation: work items grouped by their supertile's
+// order (the ranges read back with the far field's order), each group
+// launched with the kernel of its own order on the expansions of the far
+// field's order P (row strides of order P).
 void m2l_by_class(Tree& t, int mode, int P, hipStream_t st) {
-  int start[M2LSchedule::kMaxOrder + 1];
-  chunk_classes(t, st, start);
+  const int* start = t.class_start_h;
   const int ms = msize(P, SO), ls = lsize(P, SO);
   for (int q = PMIN; q <= P && q < M2LSchedule::kMaxOrder; ++q) {
     const int nch = start[q + 1] - start[q];
@@ -474,14 +507,15 @@ constexpr int kJudgeTargets = 1024;
 constexpr int kJudgeDistrust = 64;
 
 // Judge sample and field scale: FP64 direct sums at kJudgeTargets
-// stratified sorted slots (t.judge_Fd) and t.judge_scale, their rms |F|
-// without the largest 0.1% (at least the largest: a near-coincident pair in
-// the sample would dominate the rms).  Needs the gathered sources t.pS.
+// stratified sorted slots (t.judge_Fd) and F_s, their rms |F| without the
+// largest 0.1% (at least the largest: a near-coincident pair in the sample
+// would dominate the rms), into t.lw[kLwScale] on the device.  Needs the
+// gathered sources t.pS.
 void field_scale(Tree& t, hipStream_t st) {
   constexpr int FO = OUT == 4 ? 1 : 0;
   const int n = (int)t.n;
   const int J = std::min(kJudgeTargets, n);
-  if (t.judge_n != J) {
+  if (t.judge_n != J) {  // once per tree
     std::vector<int> h(J);
     for (int i = 0; i < J; ++i) h[i] = (int)(((long long)(2 * i + 1) * n) / (2LL * J));
     t.judge_slot.alloc(std::max(J, 1));
@@ -490,45 +524,45 @@ void field_scale(Tree& t, hipStream_t st) {
     t.judge_n = J;
   }
   t.judge_Fd.alloc((size_t)std::max(J, 1) * OUT);
-  t.judge_scale = 0.0;
-  if (J == 0) return;
-  k_fill_zero<<<nblk((long long)J * OUT, 256), 256, 0, st>>>((long long)J * OUT, t.judge_Fd.get());
-  const dim3 g((J + kJudgeTile - 1) / kJudgeTile, (n + kJudgeSlice - 1) / kJudgeSlice);
-  k_judge_direct<<<g, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
-                                    t.judge_Fd.get());
-  std::vector<double> h((size_t)J * OUT);
-  FMM_HIP(hipMemcpyAsync(h.data(), t.judge_Fd.get(), sizeof(double) * J * OUT, hipMemcpyDeviceToHost, st));
-  FMM_HIP(hipStreamSynchronize(st));
-  std::vector<double> m2(J);
-  for (int i = 0; i < J; ++i) {
-    double a = 0.0;
-    for (int k = FO; k < OUT; ++k) a += h[(size_t)i * OUT + k] * h[(size_t)i * OUT + k];
-    m2[i] = a;
+  learned_scalars(t);
+  if (J > 0) {
+    k_fill_zero<<<nblk((long long)J * OUT, 256), 256, 0, st>>>((long long)J * OUT, t.judge_Fd.get());
+    const dim3 g((J + kJudgeTile - 1) / kJudgeTile, (n + kJudgeSlice - 1) / kJudgeSlice);
+    k_judge_direct<<<g, 256, 0, st>>>(n, t.px.get(), t.py.get(), t.pz.get(), t.pS.get(), t.judge_slot.get(), J,
+                                      t.judge_Fd.get());
   }
-  std::sort(m2.begin(), m2.end());
-  const int keep = J - std::max(1, J / 1000);
-  double sum = 0.0;
-  for (int i = 0; i < keep; ++i) sum += m2[i];
-  t.judge_scale = keep > 0 ? sqrt(sum / keep) : 0.0;
+  field_scale_from_samples(t, J, OUT, FO, st);
 }
 
 // The learned cell orders for the tree's current sources (t.cell_q),
 // without an evaluation: the order hints of the M2L tiling.
 void predict_cell_orders(Tree& t, hipStream_t st) {
-  const int n = (int)t.n, nl = t.nleaves;
-  t.pS.alloc((size_t)std::max(n, 1) * SS);
-  k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+  const int nl = t.nleaves;
+  gather_sources(t, st);
   field_scale(t, st);
   leaf_features(t, SS, st);
-  t.leaf_q.alloc(std::max(nl, 1));
   t.leaf_y.alloc((size_t)std::max(nl, 1) * kMlpO);
   mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), t.leaf_y.get(), nullptr, 0.f, 0.f, 0, 0, st);
-  allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, std::max(PMIN, kMlpOrder0), t.order, st);
-  orders_from_leaves(t, st);
+  select_orders(t, t.leaf_y.get(), t.margin, std::max(PMIN, kMlpOrder0), t.order, false, st);
 }
 
+// Learned-order evaluation: the judge sample's FP64 direct sums and field
+// scale, leaf features, FP16 network inference -> leaf orders (the
+// cheapest whose predicted norm-wise error meets the tolerance) -> cell
+// and supertile orders, FP32 P2P (on the second stream from the start),
+// mixed-precision far field at the largest order with every supertile
+// truncated at its own order, then the judge's check at its kJudgeTargets
+// targets.  The order selection runs on the device; the host waits once
+// for the far field's order and once for the judge's verdict.  When the
+// sample's relative field error exceeds the tolerance, the leaves holding
+// a target whose error exceeds tol * rms |F| rerun at the tree's order and
+// the other leaves are re-allocated for half the tolerance (all leaves run
+// at the tree's order when more than one target in kJudgeDistrust fails,
+// or when the partial rerun still misses).
+// mode 3: FP32 M2L, mode 4: BF16x3 M2L.
 void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   FMM_CHECK(t.has_model, kInvalidArgument, "no order model set on the tree");
+  t.ml_order = -1;
   const int n = (int)t.n, nl = t.nleaves;
   const int fmode = mode == 4 ? 2 : 1;
   const int qmin = std::max(PMIN, kMlpOrder0);
@@ -547,35 +581,14 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   {
     PhaseTimer pt(t, kTOrder, st);
     leaf_features(t, SS, st);
-    t.leaf_q.alloc(std::max(nl, 1));
     t.leaf_y.alloc((size_t)std::max(nl, 1) * kMlpO);
     mlp_infer_f16(nl, t.leaf_feat.get(), t.mlp_half.get(), t.leaf_y.get(), nullptr, 0.f, 0.f, 0, 0, st);
-    allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, qmin, t.order, st);
-    peff = orders_from_leaves(t, st);
-    // the tile maxima leave the predicted error below the budget: allocate
-    // once more for the tolerance scaled by that slack, keep it when its
-    // effective error still meets the tolerance
-    double e1 = effective_error(t, t.leaf_y.get(), t.margin, t.order, st);
-    if (e1 > 0.0 && e1 < 0.9 * t.tol) {
-      const double tol2 = std::min(t.tol * t.tol / e1, 4.0 * t.tol);
-      allocate_orders(t, t.leaf_y.get(), tol2, t.margin, qmin, t.order, st);
-      const int p2 = orders_from_leaves(t, st);
-      const double e2 = effective_error(t, t.leaf_y.get(), t.margin, t.order, st);
-      if (e2 <= t.tol) {
-        peff = p2;
-        e1 = e2;
-      } else {
-        allocate_orders(t, t.leaf_y.get(), t.tol, t.margin, qmin, t.order, st);
-        peff = orders_from_leaves(t, st);
-      }
-    }
-    t.lstats[7] = e1;
+    select_orders(t, t.leaf_y.get(), t.margin, qmin, t.order, true, st);
+    chunk_classes_async(t, st);
+    peff = read_orders(t, st, t.class_start_h);
   }
   peff = std::max(std::min(peff, t.order), PMIN);
-  // near field (FP32, compensated); running it on a second stream beside
-  // the M2L was measured to gain nothing: both keep the compute units busy
-  near_field(t, 1, st);
-  struct Restore {  // the tile orders apply to this evaluation only
+  struct Restore {  // the supertile orders apply to this evaluation only
     Tree& t;
     ~Restore() {
       t.use_tile_order = false;
@@ -584,8 +597,13 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   } restore{t};
   t.use_tile_order = true;
   t.m2l_classes = &m2l_by_class;
+  bool near_done = false;
   auto far = [&](int p) {
     OrderTable<PMIN>::far(p, t, fmode, kPhaseAll & ~kPhaseP2P, st);
+    if (!near_done) {  // near field (FP32, compensated) once; reruns reuse it
+      near_field(t, 1, st);
+      near_done = true;
+    }
     OrderTable<PMIN>::l2p(p, t, fmode, kPhaseAll, F, st);
   };
   far(peff);
@@ -605,6 +623,12 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
     jerr = h[1] > 0 ? sqrt(h[0] / h[1]) : 0.0;
     return nf;
   };
+  // orders after a change of the leaf orders: cells, supertiles, classes
+  auto reorder = [&]() {
+    orders_from_leaves(t, t.leaf_q.get(), t.cell_q.get(), t.tile_order.get(), st);
+    chunk_classes_async(t, st);
+    return std::max(std::min(read_orders(t, st, t.class_start_h), t.order), PMIN);
+  };
   bool rerun = false;
   if (J > 0) {
     PhaseTimer pt(t, kTJudge, st);
@@ -618,25 +642,35 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
     // predictions ran optimistic); more than one in kJudgeDistrust of the
     // sample: the network is not trusted on this input and every leaf
     // reruns at the full order
-    if ((long long)fails * kJudgeDistrust > J) {
+    const bool all = (long long)fails * kJudgeDistrust > J;
+    if (all) {
       raise_all_leaves(t, t.order, st);
     } else {
-      allocate_orders(t, t.leaf_y.get(), 0.5 * t.tol, t.margin, qmin, t.order, st);
+      k_set_scalar<<<1, 64, 0, st>>>(t.lw.get() + kLwTol2, 0.5 * t.tol);
+      k_set_scalar<<<1, 64, 0, st>>>(t.lw.get() + kLwRealloc, 1.0);
+      allocate_orders(t, t.leaf_y.get(), kLwTol2, t.margin, qmin, t.order, t.leaf_q.get(), kLwPred2, false, st);
       raise_failing_leaves(t, t.order, st);
     }
-    peff = std::max(std::min(orders_from_leaves(t, st), t.order), PMIN);
+    peff = reorder();
     far(peff);
     reruns = 1;
     check();
+    if (!all && jerr > t.tol) {
+      // the partial rerun still misses: the fixed-order method at the
+      // tree's full order, every leaf
+      raise_all_leaves(t, t.order, st);
+      peff = reorder();
+      far(peff);
+      reruns = 2;
+      check();
+    }
   }
   t.lstats[0] = peff;
   t.lstats[1] = fails;
   t.lstats[2] = reruns;
   t.lstats[3] = jerr;
   t.lstats[4] = J;
-  t.lstats[5] = m2l_mac_fraction(t, t.order, st, &t.lstats[6]);
-  t.lstats[8] = t.judge_scale;
-  t.lstats[9] = m2l_cell_macs(t, st);
+  t.lstats_ready = false;  // the order statistics are computed when asked for
 }
 
 void op_batch(int op, int order, int n, const double* xyz, const double* in, int in_stride, double* out,
@@ -683,6 +717,7 @@ const Variant FMM_VARIANT_NAME = {FMM_VARIANT_NS::SO,
                                   &FMM_VARIANT_NS::evaluate_learned,
                                   &FMM_VARIANT_NS::field_scale,
                                   &FMM_VARIANT_NS::predict_cell_orders,
+                                  &FMM_VARIANT_NS::gather_sources,
                                   &FMM_VARIANT_NS::op_batch,
                                   &FMM_VARIANT_NS::direct};
 #endif

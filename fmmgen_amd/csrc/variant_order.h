@@ -132,13 +132,14 @@ const unsigned* ensure_index_table(Tree& t, hipStream_t st) {
   return S.idxtab[P][wi].get();
 }
 
-template <int P>
-const short* ensure_index_table32(Tree& t, hipStream_t st) {
+template <int P, int W>
+const unsigned* ensure_index_table32(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (!S.idx32_ready[P]) {
-    const std::vector<short> tab = m2l_index_table32<P, SO>();
+    const std::vector<unsigned> tab = m2l_index_table32<P, SO, W>();
     S.idxtab32[P].alloc(tab.size());
-    FMM_HIP(hipMemcpyAsync(S.idxtab32[P].get(), tab.data(), sizeof(short) * tab.size(), hipMemcpyHostToDevice, st));
+    FMM_HIP(hipMemcpyAsync(S.idxtab32[P].get(), tab.data(), sizeof(unsigned) * tab.size(), hipMemcpyHostToDevice,
+                           st));
     FMM_HIP(hipStreamSynchronize(st));
     S.idx32_ready[P] = true;
   }
@@ -250,8 +251,9 @@ void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hi
   if (nch == 0) return;
   const int grid = ((nch + 7) / 8) * 8;
   if (mode == 2) {  // BF16x3 split precision
-    const short* idx = ensure_index_table32<Q>(t, st);
-    k_m2l_bf16x3<Q, SO><<<grid, 64, 0, st>>>(nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
+    constexpr int W32 = m2l_waves32<Q, SO>();
+    const unsigned* idx = ensure_index_table32<Q, W32>(t, st);
+    k_m2l_bf16x3<Q, SO, W32><<<grid, 64 * W32, 0, st>>>(nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
                                              S.chunk_g1.get(), S.group_key.get(), S.group_src.get(),
                                              S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks,
                                              ms, ls, t.Mf.get(), t.Lf.get());

@@ -97,8 +97,8 @@ class Tree:
         return v.value
 
     def leaf_error(self, F, Fref):
-        """[nleaves] float64 device tensor: the largest |F - Fref| (field
-        components) over each leaf's particles."""
+        """[nleaves] float64 device tensor: the root-mean-square |F - Fref|
+        (field components) over each leaf's particles."""
         _check_dev(F, "F", self.outputsize)
         _check_dev(Fref, "Fref", self.outputsize)
         e = torch.empty(self.nleaves(), dtype=torch.float64, device="cuda")
@@ -128,7 +128,8 @@ class Tree:
         a = (ctypes.c_longlong * 16)()
         _lib.check(_lib.load().fmm_tree_info(self._h, a))
         keys = ("ncells", "nleaves", "levels", "n_m2l", "n_p2p", "n", "order", "source_order",
-                "p2p_particle_pairs", "ncrit", "m2l_tiles", "m2l_groups", "m2l_keys", "p2p_items")
+                "p2p_particle_pairs", "ncrit", "m2l_tiles", "m2l_groups", "m2l_keys", "p2p_items",
+                "m2l_chunks")
         return dict(zip(keys, list(a)))
 
     def timing(self, enable):

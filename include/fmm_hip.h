@@ -4,8 +4,10 @@
  * the current HIP device; h_* are host memory.  `stream` is a hipStream_t
  * (NULL: the default stream).  Every call returns FMM_OK or an error code;
  * fmm_last_error() describes the last failure of the calling thread.
- * Calls are stream ordered; only the tree build and the exports
- * synchronise the stream.
+ * Calls are stream ordered.  The tree build, the exports, the statistics
+ * queries and fmm_tree_set_model synchronise; a learned evaluation
+ * synchronises once, to launch its far field at the largest selected order
+ * (and once more per judge check: the judge's verdict decides a rerun).
  *
  * Reference counterparts (paths in the rpep/fmmgen tree):
  *   fmm_tree_build        build_tree(pos, S, n, ncrit, order, theta)
@@ -87,7 +89,8 @@ int fmm_tree_destroy(fmm_tree_t tree);
  * source_order, P2P particle pairs (self pairs excluded), ncrit,
  * M2L target tiles, M2L (tile, offset) groups, distinct M2L offsets;
  * info[13] = P2P work items (source segments of the target leaves);
- * info[14..15] = 0 (reserved) */
+ * info[14] = M2L work items (runs of groups of one tile);
+ * info[15] = 0 (reserved) */
 #define FMM_TREE_INFO_LEN 16
 int fmm_tree_info(fmm_tree_t tree, long long* info);
 
@@ -196,7 +199,9 @@ int fmm_tree_leaves(fmm_tree_t tree, long long* h_leaves);
 int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, double margin);
 /* Statistics of the last learned evaluation, h_stats [10]: largest order
  * used, judge targets above tol * rms |F| (a rerun happens when the
- * sample's rel. field error exceeds tol), reruns (0/1), judge rel. field
+ * sample's rel. field error exceeds tol), reruns (0: none, 1: the
+ * failing leaves (or all leaves) at the full order, 2: a partial rerun
+ * that still missed tol, then every leaf at the full order), judge rel. field
  * error (after the rerun, if any), judge targets, M2L multiply-adds
  * relative to the tree's order, M2L useful multiply-adds (sum over pairs of
  * C(q + 6, 6) at the pair's order), predicted error of the chosen orders
