@@ -20,6 +20,7 @@ MODES = {"plain": MODE_PLAIN_F64, "f32": MODE_MIXED_F32, "bf16x3": MODE_MIXED_BF
          "learned": MODE_LEARNED_F32, "learned_bf16x3": MODE_LEARNED_BF16X3}
 
 MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_ORDER0, MLP_PARAMS = 12, 64, 11, 2, 5707
+MLP_NOALLOC_MAX_BATCH = 8192
 
 PHASE_P2M, PHASE_M2M, PHASE_M2L, PHASE_P2P, PHASE_L2L, PHASE_L2P = 1, 2, 4, 8, 16, 32
 PHASE_ALL = 63
@@ -32,7 +33,7 @@ EXPORTS = (
     "fmm_tree_info", "fmm_tree_set_sources", "fmm_evaluate", "fmm_evaluate_phases", "fmm_direct_sampled",
     "fmm_op_batch", "fmm_export_cells", "fmm_export_particles", "fmm_export_lists", "fmm_export_expansions",
     "fmm_timing_enable", "fmm_timing_read",
-    "fmm_mlp_forward", "fmm_mlp_infer_f16", "fmm_mlp_work_floats", "fmm_mlp_grad", "fmm_mlp_adam",
+    "fmm_mlp_forward", "fmm_mlp_infer_f16", "fmm_mlp_work_floats", "fmm_mlp_grad", "fmm_mlp_grad_noalloc", "fmm_mlp_adam",
     "fmm_tree_set_order", "fmm_tree_leaf_features", "fmm_tree_leaf_error", "fmm_tree_leaves",
     "fmm_tree_set_model", "fmm_tree_learned_stats", "fmm_tree_leaf_orders", "fmm_tree_field_scale",
 )
@@ -92,6 +93,7 @@ def load(path=None):
     L.fmm_mlp_infer_f16.argtypes = [vp, ci, vp, vp, vp, cd, cd, ci, ci, vp]
     L.fmm_mlp_work_floats.argtypes = [ci]
     L.fmm_mlp_grad.argtypes = [vp, vp, vp, ci, vp, vp, vp, vp, vp]
+    L.fmm_mlp_grad_noalloc.argtypes = [vp, vp, vp, ci, vp, vp, vp, vp]
     L.fmm_mlp_adam.argtypes = [vp, vp, vp, vp, cf, cf, cf, cf, ci, vp]
     L.fmm_tree_set_order.argtypes = [vp, ci]
     L.fmm_tree_leaf_features.argtypes = [vp, vp, vp]

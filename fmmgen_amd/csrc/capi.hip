@@ -370,6 +370,18 @@ int fmm_mlp_grad(const float* d_X, const float* d_T, const float* d_mask, int B,
   });
 }
 
+int fmm_mlp_grad_noalloc(const float* d_X, const float* d_T, const float* d_mask, int B, const float* d_params,
+                         float* d_grad, float* d_loss, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(d_X && d_T && d_mask && d_params && d_grad && d_loss, kInvalidArgument, "NULL argument");
+    FMM_CHECK(B <= kMlpScratchBatch, kInvalidArgument,
+              "batch of " + std::to_string(B) + " samples exceeds the fixed workspace (" +
+                  std::to_string(kMlpScratchBatch) + " samples at most)");
+    mlp_grad(B, d_X, d_T, d_mask, d_params, d_grad, d_loss, mlp_scratch(), static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
+}
+
 int fmm_mlp_adam(const float* d_grad, float* d_params, float* d_m, float* d_v, float lr, float beta1, float beta2,
                  float eps, int step, void* stream) {
   return guarded([&] {

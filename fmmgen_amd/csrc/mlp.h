@@ -22,7 +22,11 @@ constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpH * kMlpF, kMlpW2 = kMlpB1 + kMl
               kMlpW3 = kMlpB2 + kMlpH, kMlpB3 = kMlpW3 + kMlpO * kMlpH, kMlpParams = kMlpB3 + kMlpO;
 
 // Workspace of one training batch of B samples (device floats).
-inline size_t mlp_work_floats(int B) { return (size_t)B * (2 * kMlpH + kMlpO) * 2 + 1; }
+constexpr size_t mlp_work_floats(int B) { return (size_t)B * (2 * kMlpH + kMlpO) * 2 + 1; }
+
+// Largest batch of the allocation-free gradient (its workspace is a fixed
+// device array of mlp_work_floats(kMlpScratchBatch) floats, 9.1 MB).
+constexpr int kMlpScratchBatch = 8192;
 
 // FP32 forward: Y[B][O].
 void mlp_forward(int B, const float* X, const float* params, float* Y, hipStream_t st);
@@ -30,6 +34,10 @@ void mlp_forward(int B, const float* X, const float* params, float* Y, hipStream
 // with respect to all parameters: grad[kMlpParams], loss[1] (device).
 void mlp_grad(int B, const float* X, const float* T, const float* mask, const float* params, float* grad,
               float* loss, float* work, hipStream_t st);
+// The fixed workspace on the current device (its address is looked up on
+// the first call per device; nothing is allocated).  Every call on one
+// device shares it: gradients that overlap on one device race.
+float* mlp_scratch();
 // One Adam step (bias-corrected, step >= 1).
 void mlp_adam(const float* grad, float* params, float* m, float* v, float lr, float beta1, float beta2, float eps,
               int step, hipStream_t st);

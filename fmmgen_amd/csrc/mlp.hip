@@ -1,6 +1,8 @@
 // Kernels of the order-selection network (layout and semantics in mlp.h).
 #include "mlp.h"
 
+#include <mutex>
+
 namespace fmm {
 namespace {
 
@@ -11,8 +13,10 @@ __device__ __forceinline__ void stage_params(const float* __restrict__ P, float*
   __syncthreads();
 }
 
-// Sum of the mask (the number of valid outputs of the batch) into cnt[0].
-__global__ void k_mask_count(long long n, const float* __restrict__ mask, float* __restrict__ cnt) {
+// Sum of the mask (the number of valid outputs of the batch) into cnt[0];
+// loss[0] cleared for the forward pass that adds into it.
+__global__ void k_mask_count(long long n, const float* __restrict__ mask, float* __restrict__ cnt,
+                             float* __restrict__ loss) {
   __shared__ float red[256];
   float s = 0.f;
   for (long long i = threadIdx.x; i < n; i += blockDim.x) s += mask[i] != 0.f ? 1.f : 0.f;
@@ -22,7 +26,10 @@ __global__ void k_mask_count(long long n, const float* __restrict__ mask, float*
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) cnt[0] = red[0];
+  if (threadIdx.x == 0) {
+    cnt[0] = red[0];
+    loss[0] = 0.f;
+  }
 }
 
 // Forward pass, one thread per sample.  With T: the masked mean-square loss
@@ -207,7 +214,31 @@ __global__ __launch_bounds__(64) void k_mlp_infer(int B, const float* __restrict
 
 inline int nb(long long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// Workspace of the allocation-free gradient.  In host code `scratch` names
+// a host-side stand-in, never the device array: host code uses only the
+// per-device address from hipGetSymbolAddress.  Nothing in it has to start
+// at zero (every kernel writes what it later reads).
+static __device__ __align__(256) float scratch[mlp_work_floats(kMlpScratchBatch)];
+
+constexpr int kMaxDevices = 64;
+
 }  // namespace
+
+float* mlp_scratch() {
+  static std::mutex mu;
+  static float* addr[kMaxDevices] = {};
+  int dev = 0;
+  FMM_HIP(hipGetDevice(&dev));
+  FMM_CHECK(dev >= 0 && dev < kMaxDevices, kUnsupported, "device index beyond the workspace table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    FMM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FMM_CHECK(p != nullptr, kHipError, "no device address for the gradient workspace");
+    addr[dev] = static_cast<float*>(p);
+  }
+  return addr[dev];
+}
 
 void mlp_forward(int B, const float* X, const float* params, float* Y, hipStream_t st) {
   if (B <= 0) return;
@@ -225,8 +256,7 @@ void mlp_grad(int B, const float* X, const float* T, const float* mask, const fl
   float* D2 = dY + (size_t)B * O;
   float* D1 = D2 + (size_t)B * H;
   float* cnt = D1 + (size_t)B * H;
-  k_mask_count<<<1, 256, 0, st>>>((long long)B * O, mask, cnt);
-  FMM_HIP(hipMemsetAsync(loss, 0, sizeof(float), st));
+  k_mask_count<<<1, 256, 0, st>>>((long long)B * O, mask, cnt, loss);
   k_mlp_fwd<<<nb(B, 64), 64, 0, st>>>(B, X, params, H1, H2, Y, T, mask, cnt, dY, loss);
   k_mlp_bwd<<<nb(B, 64), 64, 0, st>>>(B, params, H1, H2, dY, D2, D1);
   k_mlp_wgrad<<<nb(O * (H + 1), 128), 128, 0, st>>>(B, O, H, dY, H2, grad + kMlpW3, grad + kMlpB3);

@@ -170,6 +170,17 @@ int fmm_mlp_infer_f16(const float* d_X, int B, const float* d_params, float* d_Y
 size_t fmm_mlp_work_floats(int B);
 int fmm_mlp_grad(const float* d_X, const float* d_T, const float* d_mask, int B, const float* d_params,
                  float* d_grad, float* d_loss, float* d_work, void* stream);
+/* fmm_mlp_grad without the workspace argument and without any device
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * training loops).  The workspace is a fixed array inside the library, one
+ * per device, of fmm_mlp_work_floats(FMM_MLP_NOALLOC_MAX_BATCH) floats
+ * (9.1 MB, held until the process ends); B above FMM_MLP_NOALLOC_MAX_BATCH
+ * is FMM_ERR_INVALID_ARGUMENT.  Calls that overlap on one device share the
+ * array and race; calls ordered on one stream do not. */
+#define FMM_MLP_NOALLOC_MAX_BATCH 8192
+int fmm_mlp_grad_noalloc(const float* d_X, const float* d_T, const float* d_mask, int B, const float* d_params,
+                         float* d_grad, float* d_loss, void* stream);
 /* One bias-corrected Adam step (step >= 1) on the parameters. */
 int fmm_mlp_adam(const float* d_grad, float* d_params, float* d_m, float* d_v, float lr, float beta1, float beta2,
                  float eps, int step, void* stream);

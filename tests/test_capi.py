@@ -50,6 +50,10 @@ def test_argument_errors_are_reported():
     assert L.fmm_op_batch(99, 4, 0, 4, 1, fake, fake, 1, fake, 1, None) == 1
     assert L.fmm_op_batch(1, 4, 0, 4, 0, None, None, 1, None, 1, None) == _lib.FMM_OK   # empty batch
     assert L.fmm_tree_destroy(None) == _lib.FMM_OK
+    big = _lib.MLP_NOALLOC_MAX_BATCH + 1
+    assert L.fmm_mlp_grad_noalloc(fake, fake, fake, big, fake, fake, fake, None) == 1
+    assert b"exceeds the fixed workspace" in L.fmm_last_error()
+    assert L.fmm_mlp_grad_noalloc(None, fake, fake, 8, fake, fake, fake, None) == 1
 
 
 def test_python_layer_rejects_host_tensors():
