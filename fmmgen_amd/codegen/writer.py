@@ -17,8 +17,12 @@ Differences, by design:
 * The bodies are straight-line programs whose common subexpressions are
   shared by construction (see ``ops``), so ``CSE`` and ``minpow`` do not
   change them: no ``pow()`` call is ever printed.
-* ``cython=True`` is not supported: Python reaches the compiled operators
-  through ctypes (``fmmgen_amd.operators``).
+* ``cython=True`` writes ``<name>_wrap.py`` instead of the Cython files
+  (fmmgen/writer.py:346-432): a Python module with the same names (the
+  ``FMMGEN_*`` constants, ``P2M(..., order)`` ... ``P2P``, ``P2M_<p>`` ...)
+  bound to the compiled HIP operators (``fmmgen_amd.operators``), for the
+  generated order range.  With ``gpu=True`` it raises ``Warning`` as the
+  reference does.
 * The extra device operators ``P2M_S_<p>`` (the example's P2M through M2M,
   example/calculate.cpp:135-161) and ``Dgen_<p>`` (the derivative block of
   M2L) are emitted with ``gpu=True``; they have no counterpart in the
@@ -51,6 +55,10 @@ def build_programs(order, source_order=0, potential=True, field=True,
             progs.append(ops.gen_Dblock(i, source_order, harmonic_derivs))
         if i == start:
             progs.append(ops.gen_P2P(source_order, potential, field, harmonic=harmonic_derivs))
+    if gpu_extras:
+        # one derivative block beyond the last order: the field of M2P at
+        # the last order needs derivatives of order + 1
+        progs.append(ops.gen_Dblock(order, source_order, harmonic_derivs))
     return start, progs
 
 
@@ -123,6 +131,7 @@ def device_source(order, potential=True, field=True, source_order=0,
     pr = HipPrinter()
     text = _macros(start, order, source_order, potential, field)
     text += f"#define FMMGEN_FULL_BODIES_MAXORDER {full_bodies_max_order}\n"
+    text += f"#define FMMGEN_DGEN_MAXORDER {order}\n"
     text += f"#define FMMGEN_CHUNK {CHUNK}\n" + pr.prelude()
     text += "namespace fmmgen_ops {\n"
     wrappers = {}
@@ -200,8 +209,6 @@ def generate_code(order, name, precision="double", cython=False, CSE=False,
         raise AssertionError("Language must be 'c' or 'c++'")
     if precision not in ("double", "float"):
         raise AssertionError("Precision must be float or double")
-    if cython:
-        raise NotImplementedError("Cython wrappers are not generated; use fmmgen_amd.operators (ctypes)")
     if order <= source_order + (1 if field else 0):
         raise ValueError("order must exceed the first generated order")
     logger.info("Generating FMM operators to order %d (CSE shared by construction: %s, minpow %s)",
@@ -212,6 +219,8 @@ def generate_code(order, name, precision="double", cython=False, CSE=False,
         text = device_source(order, potential, field, source_order, harmonic_derivs)
         with open(os.path.join(inc_dir, f"{name}.h"), "w") as f:
             f.write(text)
+        if cython:
+            raise Warning("Cannot write a Cython wrapper for GPU code; skipping")
         return
     header, source, counts = host_source(order, name, precision, potential, field,
                                          source_order, atomic, harmonic_derivs, language)
@@ -224,3 +233,21 @@ def generate_code(order, name, precision="double", cython=False, CSE=False,
         with open(save_opscounts, "w") as f:
             for nm, c in counts:
                 f.write(f"{nm},{c}\n")
+    if cython:
+        start = source_order + (1 if field else 0)
+        logger.info("Generating Python operator module: %s_wrap.py", name)
+        with open(f"{name}_wrap.py", "w") as f:
+            f.write(wrapper_module(name, start, order, source_order, potential, field))
+
+
+def wrapper_module(name, start, order, source_order, potential, field):
+    """Text of ``<name>_wrap.py``: the reference's Cython module names bound
+    to the compiled HIP operators for orders start..order-1."""
+    return (f'"""Operators {name}: orders {start}..{order - 1}, source order {source_order}, '
+            f'potential={potential}, field={field}.\n\n'
+            "Written by fmmgen_amd generate_code(..., cython=True); the functions run the\n"
+            'compiled HIP operators of libfmm_hip.so (fmmgen_amd.operators)."""\n'
+            "from fmmgen_amd.operators import Library as _Library\n\n"
+            f"_lib = _Library(source_order={source_order}, potential={potential}, field={field}, "
+            f"min_order={start}, max_order={order})\n"
+            "globals().update(_lib.functions())\n")

@@ -504,7 +504,9 @@ void m2l_by_class(Tree& t, int mode, int P, hipStream_t st) {
 }
 
 constexpr int kJudgeTargets = 1024;
-constexpr int kJudgeDistrust = 64;
+// A sample error above kJudgeDistrust times the tolerance: the network is
+// not trusted on this input.
+constexpr double kJudgeDistrust = 4.0;
 
 // Judge sample and field scale: FP64 direct sums at kJudgeTargets
 // stratified sorted slots (t.judge_Fd) and F_s, their rms |F| without the
@@ -549,16 +551,16 @@ void predict_cell_orders(Tree& t, hipStream_t st) {
 // Learned-order evaluation: the judge sample's FP64 direct sums and field
 // scale, leaf features, FP16 network inference -> leaf orders (the
 // cheapest whose predicted norm-wise error meets the tolerance) -> cell
-// and supertile orders, FP32 P2P (on the second stream from the start),
-// mixed-precision far field at the largest order with every supertile
-// truncated at its own order, then the judge's check at its kJudgeTargets
-// targets.  The order selection runs on the device; the host waits once
-// for the far field's order and once for the judge's verdict.  When the
-// sample's relative field error exceeds the tolerance, the leaves holding
-// a target whose error exceeds tol * rms |F| rerun at the tree's order and
-// the other leaves are re-allocated for half the tolerance (all leaves run
-// at the tree's order when more than one target in kJudgeDistrust fails,
-// or when the partial rerun still misses).
+// and supertile orders, mixed-precision far field at the largest order
+// with every supertile truncated at its own order, FP32 P2P, then the
+// judge's check at its kJudgeTargets targets.  The order selection runs on
+// the device; the host waits once for the far field's order and once for
+// the judge's verdict.  When the sample's relative field error e exceeds
+// the tolerance, the leaves holding a target whose error exceeds
+// tol * rms |F| rerun at the tree's order and the other leaves are
+// re-allocated for 0.5 tol^2 / e (the predictions ran optimistic by about
+// e / tol); all leaves run at the tree's order when e exceeds
+// kJudgeDistrust * tol, or when the partial rerun still misses.
 // mode 3: FP32 M2L, mode 4: BF16x3 M2L.
 void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   FMM_CHECK(t.has_model, kInvalidArgument, "no order model set on the tree");
@@ -637,16 +639,15 @@ void evaluate_learned(Tree& t, int mode, double* F, hipStream_t st) {
   }
   if (rerun) {
     PhaseTimer pt(t, kTJudge, st);
-    // a few failures: their leaves rerun at the full order and the other
-    // leaves are re-allocated for half the tolerance (the sample says the
-    // predictions ran optimistic); more than one in kJudgeDistrust of the
-    // sample: the network is not trusted on this input and every leaf
-    // reruns at the full order
-    const bool all = (long long)fails * kJudgeDistrust > J;
+    // a near miss: the failing targets' leaves rerun at the full order and
+    // the other leaves are re-allocated for a tolerance lowered by the
+    // factor the sample missed by, and half again; a wide miss: the network
+    // is not trusted on this input and every leaf reruns at the full order
+    const bool all = jerr > kJudgeDistrust * t.tol;
     if (all) {
       raise_all_leaves(t, t.order, st);
     } else {
-      k_set_scalar<<<1, 64, 0, st>>>(t.lw.get() + kLwTol2, 0.5 * t.tol);
+      k_set_scalar<<<1, 64, 0, st>>>(t.lw.get() + kLwTol2, 0.5 * t.tol * (t.tol / jerr));
       k_set_scalar<<<1, 64, 0, st>>>(t.lw.get() + kLwRealloc, 1.0);
       allocate_orders(t, t.leaf_y.get(), kLwTol2, t.margin, qmin, t.order, t.leaf_q.get(), kLwPred2, false, st);
       raise_failing_leaves(t, t.order, st);

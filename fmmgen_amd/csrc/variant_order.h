@@ -316,15 +316,120 @@ void l2p_final_order(Tree& t, int mode, int mask, double* F, hipStream_t st) {
                                                (mask & kPhaseL2P) ? 1 : 0, F);
 }
 
+// Operators above the full-body order (their whole bodies are not
+// compiled there).  P2M, M2M and L2L: the chunked bodies, one thread per
+// (item, output chunk) as in the tree kernels; the charge P2M is the
+// example's P2M_S at the opposite displacement (q (-x)^n / n!).
+template <int P, int OPID>
+__global__ __launch_bounds__(64) void k_op_batch_chunked(int n, const double* __restrict__ xyz,
+                                                         const double* __restrict__ in, int in_stride,
+                                                         double* __restrict__ out, int out_stride) {
+  constexpr int NOUT = OPID == kOpL2L ? Op<P>::LS : Op<P>::MS, NCH = (NOUT + kChunk - 1) / kChunk;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  const double* a = in + (size_t)i * in_stride;
+  double* o = out + (size_t)i * out_stride;
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < NOUT) ? LO + kChunk : NOUT;
+    ChunkAcc<double, LO, HI> acc;
+#pragma unroll
+    for (int k = 0; k < HI - LO; ++k) acc.v[k] = 0.0;
+    if constexpr (OPID == kOpM2M) fmmgen_ops::M2M_chunk<P, LO / kChunk, double>(x, y, z, a, acc);
+    if constexpr (OPID == kOpL2L) fmmgen_ops::L2L_chunk<P, LO / kChunk, double>(x, y, z, a, acc);
+    if constexpr (OPID == kOpP2MS) fmmgen_ops::P2M_S_chunk<P, LO / kChunk, double>(x, y, z, a, acc);
+    if constexpr (OPID == kOpP2M) {
+      static_assert(SO == 0, "charge P2M through P2M_S needs source order 0");
+      fmmgen_ops::P2M_S_chunk<P, LO / kChunk, double>(-x, -y, -z, a, acc);
+    }
+#pragma unroll
+    for (int k = 0; k < HI - LO; ++k) o[LO + k] += acc.v[k];
+  });
+}
+
+// M2L and M2P above the full-body order, one block per item: the generated
+// derivative block (orders up to P for M2L, P + 1 for M2P's field) is
+// staged in LDS and the sums run over the multi-index arithmetic of the
+// graded layout:
+//   M2L  L_n += sum_m D_{n+m} M_m,  |n| + |m| <= P
+//   M2P  phi += sum_m M_m D_m,  F_a -= sum_m M_m D_{m+e_a}
+template <int P, int OPID>
+__global__ __launch_bounds__(64) void k_op_batch_dsum(int n, const double* __restrict__ xyz,
+                                                      const double* __restrict__ in, int in_stride,
+                                                      double* __restrict__ out, int out_stride) {
+  constexpr int MS = Op<P>::MS, LS = Op<P>::LS, OFF = nterms(SO - 1);
+  constexpr int PD = OPID == kOpM2P && OUT != 1 ? P + 1 : P;  // derivative order
+  constexpr int DS = msize(PD, SO);
+  __shared__ double sD[DS];
+  __shared__ double sM[MS];
+  const int i = blockIdx.x;
+  if (i >= n) return;
+  const double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+  const double* a = in + (size_t)i * in_stride;
+  double* o = out + (size_t)i * out_stride;
+  if (threadIdx.x == 0) {
+    double D[DS];
+    fmmgen_ops::Dgen_o<PD, double>(x, y, z, D);
+    for (int j = 0; j < DS; ++j) sD[j] = D[j];
+  }
+  for (int j = threadIdx.x; j < MS; j += blockDim.x) sM[j] = a[j];
+  __syncthreads();
+  if constexpr (OPID == kOpM2L) {
+    for (int r = threadIdx.x; r < LS; r += blockDim.x) {
+      const MultiIdx na = slot_multi(r);
+      const int dn = na.x + na.y + na.z;
+      double acc = 0.0;
+      for (int m = 0; m < MS; ++m) {
+        const MultiIdx mb = slot_multi(m + OFF);
+        if (dn + mb.x + mb.y + mb.z > P) break;  // M slots ascend in degree
+        acc += sD[multi_slot(na.x + mb.x, na.y + mb.y, na.z + mb.z) - OFF] * sM[m];
+      }
+      o[r] += acc;
+    }
+  } else {
+    // outputs: potential (when present), then the three field components
+    constexpr bool POT = OUT != 3;
+    const int j = threadIdx.x;
+    if (j < OUT) {
+      const int ax = POT ? j - 1 : j;  // -1: potential
+      double acc = 0.0;
+      for (int m = 0; m < MS; ++m) {
+        const MultiIdx mb = slot_multi(m + OFF);
+        const int k = multi_slot(mb.x + (ax == 0), mb.y + (ax == 1), mb.z + (ax == 2)) - OFF;
+        acc += sM[m] * sD[k];
+      }
+      o[j] += ax < 0 ? acc : -acc;
+    }
+  }
+}
+
 template <int P>
 void op_batch_order(int op, int n, const double* xyz, const double* in, int is, double* out, int os,
                     hipStream_t st) {
   const int g = nblk(n, 64);
   if constexpr (P > FMMGEN_FULL_BODIES_MAXORDER) {
     // whole-operator bodies are compiled up to FMMGEN_FULL_BODIES_MAXORDER
-    if (op != kOpL2P) throw Error{kInvalidArgument, "operator batch above the full-body order: only L2P"};
-    k_op_batch<P, kOpL2P><<<g, 64, 0, st>>>(n, xyz, in, is, out, os);
-    return;
+    constexpr int NM = (Op<P>::MS + kChunk - 1) / kChunk, NL = (Op<P>::LS + kChunk - 1) / kChunk;
+    switch (op) {
+      case kOpP2M:
+        if constexpr (SO == 0) {
+          k_op_batch_chunked<P, kOpP2M><<<dim3(g, NM), 64, 0, st>>>(n, xyz, in, is, out, os);
+          return;
+        }
+        throw Error{kUnsupported, "charge P2M above the full-body order needs source order 0"};
+      case kOpP2MS: k_op_batch_chunked<P, kOpP2MS><<<dim3(g, NM), 64, 0, st>>>(n, xyz, in, is, out, os); return;
+      case kOpM2M: k_op_batch_chunked<P, kOpM2M><<<dim3(g, NM), 64, 0, st>>>(n, xyz, in, is, out, os); return;
+      case kOpL2L: k_op_batch_chunked<P, kOpL2L><<<dim3(g, NL), 64, 0, st>>>(n, xyz, in, is, out, os); return;
+      case kOpM2L: k_op_batch_dsum<P, kOpM2L><<<n, 64, 0, st>>>(n, xyz, in, is, out, os); return;
+      case kOpM2P:
+        if constexpr (P + 1 <= FMMGEN_DGEN_MAXORDER) {
+          k_op_batch_dsum<P, kOpM2P><<<n, 64, 0, st>>>(n, xyz, in, is, out, os);
+          return;
+        }
+        throw Error{kUnsupported, "M2P above the compiled derivative order"};
+      case kOpL2P: k_op_batch<P, kOpL2P><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); return;
+      default: throw Error{kInvalidArgument, "operator has no order"};
+    }
   }
   switch (op) {
     case kOpP2M: k_op_batch<P, kOpP2M><<<g, 64, 0, st>>>(n, xyz, in, is, out, os); break;

@@ -6,7 +6,11 @@ the oracle (a restatement of example/tree.cpp + example/calculate.cpp):
   * tree: bit-exact centres, half-widths, rmax, counts and leaf membership
     in canonical (octant-path) form;
   * lists: M2L and P2P lists equal as multisets;
-  * full FP64 FMM: <= 1e-12 relative to the oracle's compute_field_fmm.
+  * full FP64 FMM: <= 1e-12 relative to the oracle's compute_field_fmm;
+  * mixed FP32 / BF16x3 FMM: <= 1e-6 relative to the oracle's FP64 FMM at
+    the same order (field components);
+  * a tree reused with new strengths: the same bars against an oracle
+    built with those strengths.
 """
 import numpy as np
 import pytest
@@ -33,6 +37,9 @@ def _log(msg):
 
 @pytest.mark.parametrize("so,outsz", [(0, 4), (1, 3)])
 def test_operator_batches_match_oracle(gpu, so, outsz):
+    """Every operator at every compiled order (whole generated bodies up to
+    the full-body order; chunked bodies and the derivative-block sums
+    above it)."""
     import ctypes
     from fmmgen_amd import tree as T
     from fmmgen_amd import _lib
@@ -53,24 +60,22 @@ def test_operator_batches_match_oracle(gpu, so, outsz):
             "L2P": (xyz, rng.uniform(-1, 1, (nb, ls)), outsz, lib.oracle_op_L2P),
             "M2P": (xyz_far, rng.uniform(-1, 1, (nb, ms)), outsz, lib.oracle_op_M2P),
         }
-        if p > FULL_BODIES_MAXORDER:
-            # whole-operator batches above this order: L2P only (the
-            # chunked kernels are checked through the full FMM cases)
-            cases = {"L2P": cases["L2P"]}
-            with pytest.raises(_lib.FmmError):
-                T.op_batch("M2L", p, _dev(xyz), _dev(np.zeros((nb, ms))),
-                           torch.zeros(nb, ls, dtype=torch.float64, device="cuda"), so, outsz)
+        if so == 0:
+            cases["P2M"] = (xyz, rng.uniform(-1, 1, (nb, 1)), ms, lib.oracle_op_P2M)
         for op, (x, inp, osz, fn) in cases.items():
             _log(f"op {op} so={so} p={p}")
-            out = np.zeros((nb, osz))
+            out0 = rng.uniform(-1, 1, (nb, osz))   # the operators accumulate into their output
+            out = out0.copy()
             for i in range(nb):
                 a = np.ascontiguousarray(inp[i])
-                o = np.zeros(osz)
-                fn(x[i, 0], x[i, 1], x[i, 2], a.ctypes.data_as(O.c_double_p), o.ctypes.data_as(O.c_double_p), p)
+                o = np.ascontiguousarray(out[i])
+                if op == "P2M":
+                    fn(x[i, 0], x[i, 1], x[i, 2], a[0], o.ctypes.data_as(O.c_double_p), p)
+                else:
+                    fn(x[i, 0], x[i, 1], x[i, 2], a.ctypes.data_as(O.c_double_p), o.ctypes.data_as(O.c_double_p), p)
                 out[i] = o
-            g = T.op_batch(op, p, _dev(x), _dev(inp), torch.zeros(nb, osz, dtype=torch.float64, device="cuda"),
-                           so, outsz).cpu().numpy()
-            assert util.rel_norm(g, out) < TOL, (op, p)
+            g = T.op_batch(op, p, _dev(x), _dev(inp), _dev(out0), so, outsz).cpu().numpy()
+            assert util.rel_norm(g - out0, out - out0) < TOL, (op, p)
 
 
 # ------------------------------------------------------------------ tree + lists
@@ -91,6 +96,9 @@ def _positions(kind, n, seed):
         return util.uniform(n, seed)[0]
     if kind == "plummer":
         return util.plummer(n, seed)
+    if kind == "mixed":
+        from fmmgen_amd import workloads
+        return workloads.mixed(n, seed)
     return util.blobs(n, seed)
 
 
@@ -135,6 +143,7 @@ FMM_CASES = [
     ("uniform", 10000, 16, 3, 0.5, 1),
     ("uniform", 20000, 64, 10, 0.5, 0),
     ("blobs", 20000, 64, 12, 0.5, 0),
+    ("mixed", 30000, 128, 10, 0.5, 0),    # half uniform, half Plummer a=0.05, ncrit 128
 ]
 
 
@@ -204,25 +213,67 @@ MIXED_CASES = [
     ("uniform", 20000, 64, 8, 0.5, 0, "bf16x3"),
     ("plummer", 20000, 32, 6, 0.4, 1, "bf16x3"),
     ("blobs", 20000, 64, 12, 0.5, 0, "f32"),
+    ("blobs", 20000, 64, 12, 0.5, 0, "bf16x3"),
     ("uniform", 20000, 64, 10, 0.5, 0, "bf16x3"),
+    ("mixed", 30000, 128, 10, 0.5, 0, "bf16x3"),
+    ("mixed", 30000, 128, 10, 0.5, 0, "f32"),
 ]
 
 
+_ORACLE_FMM = {}   # oracle results shared by the two precisions of one case
+
+
 @pytest.mark.parametrize("kind,n,ncrit,order,theta,so,mode", MIXED_CASES)
-def test_fmm_mixed_close_to_plain(gpu, kind, n, ncrit, order, theta, so, mode):
-    """Mixed modes stay within 1e-6 (relative, norm-wise) of the FP64 result:
-    far below the truncation error of the expansions at these orders."""
+def test_fmm_mixed_matches_oracle(gpu, kind, n, ncrit, order, theta, so, mode):
+    """Mixed modes (FP32 or BF16x3 M2L on the matrix cores, FP32 compensated
+    P2P) stay within 1e-6 (relative, norm-wise, field components) of the
+    oracle's FP64 FMM at the same order, and of the HIP FP64 path: far
+    below the truncation error of the expansions at these orders."""
     from fmmgen_amd import build_tree
-    _log(f"mixed {mode} {kind} n={n} p={order} so={so}")
+    _log(f"mixed {mode} {kind} n={n} ncrit={ncrit} p={order} so={so}")
     pos = _positions(kind, n, 13)
     S = np.random.default_rng(14).uniform(-1, 1, (n, 1)) if so == 0 else util.dipoles(n, 14)
     outsz = 4 if so == 0 else 3
-    gt = build_tree(_dev(pos), _dev(S), n, ncrit, order, theta, source_order=so, potential=(so == 0))
-    F64 = torch.zeros(n, outsz, dtype=torch.float64, device="cuda")
-    gt.compute_field_fmm(F64, mode="plain")
-    Fm = torch.zeros_like(F64)
-    gt.compute_field_fmm(Fm, mode=mode)
     fo = 1 if so == 0 else 0
-    err = util.rel_norm(Fm.cpu().numpy()[:, fo:], F64.cpu().numpy()[:, fo:])
-    _log(f"  mixed vs plain field difference {err:.3e}")
+    key = (kind, n, ncrit, order, theta, so)
+    if key not in _ORACLE_FMM:
+        _ORACLE_FMM[key] = O.OracleTree(pos, S, ncrit, order, theta, source_order=so,
+                                        outputsize=outsz).compute_field_fmm()
+    Fo = _ORACLE_FMM[key]
+    gt = build_tree(_dev(pos), _dev(S), n, ncrit, order, theta, source_order=so, potential=(so == 0))
+    Fm = torch.zeros(n, outsz, dtype=torch.float64, device="cuda")
+    gt.compute_field_fmm(Fm, mode=mode)
+    err = util.rel_norm(Fm.cpu().numpy()[:, fo:], Fo[:, fo:])
+    _log(f"  {mode} vs oracle field difference {err:.3e}")
     assert err < 1e-6
+    F64 = torch.zeros_like(Fm)
+    gt.compute_field_fmm(F64, mode="plain")
+    assert util.rel_norm(F64.cpu().numpy(), Fo) < TOL
+    assert util.rel_norm(Fm.cpu().numpy()[:, fo:], F64.cpu().numpy()[:, fo:]) < 1e-6
+
+
+# ------------------------------------------------------------------ tree reuse
+
+def test_tree_reuse_with_new_strengths_matches_oracle(gpu):
+    """One tree, new source strengths bound with fmm_tree_set_sources (the
+    positions, tree and lists stay): plain and mixed evaluations equal the
+    oracle built with the new strengths, twice in a row."""
+    from fmmgen_amd import build_tree
+    n, ncrit, order, theta = 20000, 64, 8, 0.5
+    pos = util.blobs(n, 41)
+    rng = np.random.default_rng(42)
+    q1 = rng.uniform(-1, 1, (n, 1))
+    gt = build_tree(_dev(pos), _dev(q1), n, ncrit, order, theta)
+    F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+    gt.compute_field_fmm(F, mode="plain")
+    for k in range(2):
+        q = rng.uniform(-1, 1, (n, 1)) * (k + 2)
+        _log(f"reuse: strengths set {k}")
+        gt.set_sources(_dev(q))
+        Fo = O.OracleTree(pos, q, ncrit, order, theta).compute_field_fmm()
+        for mode, tol in (("plain", TOL), ("f32", 1e-6), ("bf16x3", 1e-6)):
+            gt.compute_field_fmm(F, mode=mode)
+            fo = 0 if mode == "plain" else 1
+            err = util.rel_norm(F.cpu().numpy()[:, fo:], Fo[:, fo:])
+            _log(f"  {mode}: {err:.3e}")
+            assert err < tol, mode
