@@ -30,7 +30,8 @@ OPS = {"P2M": 0, "M2M": 1, "M2L": 2, "L2L": 3, "L2P": 4, "M2P": 5, "P2P": 6, "P2
 # Every symbol include/fmm_hip.h declares.
 EXPORTS = (
     "fmm_last_error", "fmm_abi_version", "fmm_variant_orders", "fmm_tree_build", "fmm_tree_destroy",
-    "fmm_tree_info", "fmm_tree_set_sources", "fmm_evaluate", "fmm_evaluate_phases", "fmm_direct_sampled",
+    "fmm_tree_info", "fmm_tree_set_sources", "fmm_evaluate", "fmm_evaluate_phases", "fmm_evaluate_bh",
+    "fmm_direct_sampled",
     "fmm_op_batch", "fmm_export_cells", "fmm_export_particles", "fmm_export_lists", "fmm_export_expansions",
     "fmm_timing_enable", "fmm_timing_read",
     "fmm_mlp_forward", "fmm_mlp_infer_f16", "fmm_mlp_work_floats", "fmm_mlp_grad", "fmm_mlp_grad_noalloc", "fmm_mlp_adam",
@@ -80,6 +81,7 @@ def load(path=None):
     L.fmm_tree_set_sources.argtypes = [vp, vp]
     L.fmm_evaluate.argtypes = [vp, ci, vp, vp]
     L.fmm_evaluate_phases.argtypes = [vp, ci, ci, vp, vp]
+    L.fmm_evaluate_bh.argtypes = [vp, vp, vp]
     L.fmm_direct_sampled.argtypes = [vp, vp, sz, vp, sz, ci, ci, vp, vp]
     L.fmm_op_batch.argtypes = [ci, ci, ci, ci, sz, vp, vp, ci, vp, ci, vp]
     L.fmm_export_cells.argtypes = [vp, d_p, ll_p]

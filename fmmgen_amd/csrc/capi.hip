@@ -1,5 +1,6 @@
 // extern "C" entry points of libfmm_hip.so (declared in include/fmm_hip.h).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -110,7 +111,7 @@ int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncri
     FMM_CHECK(d_pos && d_S, kInvalidArgument, "positions and sources are required");
     FMM_CHECK(n >= 1 && n < (1u << 31), kInvalidArgument, "particle count must be in [1, 2^31)");
     FMM_CHECK(ncrit >= 1 && ncrit <= 4096, kInvalidArgument, "ncrit must be in [1, 4096]");
-    FMM_CHECK(theta > 0.0, kInvalidArgument, "theta must be positive");
+    FMM_CHECK(theta >= 0.0 && std::isfinite(theta), kInvalidArgument, "theta must be finite and non-negative");
     const Variant* v = find_variant(source_order, output_size);
     FMM_CHECK(v, kUnsupported, "no compiled variant for this source order / output size");
     FMM_CHECK(order >= v->min_order && order < v->max_order, kUnsupported,
@@ -228,6 +229,14 @@ int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, 
 
 int fmm_evaluate(fmm_tree_t tree, int mode, double* d_F, void* stream) {
   return fmm_evaluate_phases(tree, mode, FMM_PHASE_ALL, d_F, stream);
+}
+
+int fmm_evaluate_bh(fmm_tree_t tree, double* d_F, void* stream) {
+  return guarded([&] {
+    FMM_CHECK(tree && d_F, kInvalidArgument, "NULL argument");
+    tree->v->evaluate_bh(tree->t, d_F, static_cast<hipStream_t>(stream));
+    FMM_HIP(hipGetLastError());
+  });
 }
 
 int fmm_direct_sampled(const double* d_pos, const double* d_S, size_t n, const long long* d_targets, size_t nt,

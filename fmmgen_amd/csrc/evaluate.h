@@ -39,6 +39,7 @@ struct Variant {
                    int out_stride, hipStream_t);
   void (*direct)(int n, const double* pos, const double* S, const long long* targets, int nt, double* F,
                  hipStream_t);
+  void (*evaluate_bh)(Tree&, double* F, hipStream_t);
 };
 
 extern const Variant variant_so0_pf;  // charges, potential + field

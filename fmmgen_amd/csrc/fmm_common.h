@@ -242,6 +242,33 @@ struct Tree {
   long long phase_count[kNumPhaseTimers] = {0};
 };
 
+// Correctly rounded FP64 square root: the hardware sequence, then the
+// neighbour with the smallest exact residual s - c*c (one fma each).
+__device__ __forceinline__ double sqrt_rn(double s) {
+  const double r = __builtin_sqrt(s);
+  if (!(s > 0.0) || !isfinite(s) || !(r > 0.0)) return r;
+  // neighbours of a positive finite double are one bit pattern away
+  const long long bits = __double_as_longlong(r);
+  const double lo = __longlong_as_double(bits - 1);
+  const double hi = __longlong_as_double(bits + 1);
+  const double er = fabs(__builtin_fma(-r, r, s));
+  const double el = fabs(__builtin_fma(-lo, lo, s));
+  const double eh = fabs(__builtin_fma(-hi, hi, s));
+  double best = r, be = er;
+  if (el < be) { best = lo; be = el; }
+  if (eh < be) { best = hi; }
+  return best;
+}
+
+// |(dx, dy, dz)| as the reference's host code computes it:
+// sqrt(dx*dx + dy*dy + dz*dz) with every operation rounded on its own (no
+// contraction) and a correctly rounded square root, so that an opening
+// criterion on it decides as the reference does.
+__device__ inline double norm3_rn(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+  return sqrt_rn(dx * dx + dy * dy + dz * dz);
+}
+
 // Blocks of the persistent near-field kernel: kP2PWavesPerSimd one-wave
 // blocks per SIMD of every compute unit.
 constexpr int kP2PWavesPerSimd = 8;

@@ -38,24 +38,6 @@ namespace {
 
 constexpr int kMaxSplitLevels = 21;  // 3 key bits per level in 63 bits
 
-// Correctly rounded FP64 square root: the hardware sequence, then the
-// neighbour with the smallest exact residual s - c*c (one fma each).
-__device__ __forceinline__ double sqrt_rn(double s) {
-  const double r = __builtin_sqrt(s);
-  if (!(s > 0.0) || !isfinite(s) || !(r > 0.0)) return r;
-  // neighbours of a positive finite double are one bit pattern away
-  const long long bits = __double_as_longlong(r);
-  const double lo = __longlong_as_double(bits - 1);
-  const double hi = __longlong_as_double(bits + 1);
-  const double er = fabs(__builtin_fma(-r, r, s));
-  const double el = fabs(__builtin_fma(-lo, lo, s));
-  const double eh = fabs(__builtin_fma(-hi, hi, s));
-  double best = r, be = er;
-  if (el < be) { best = lo; be = el; }
-  if (eh < be) { best = hi; }
-  return best;
-}
-
 __device__ __forceinline__ double rmax_of(double r) { return sqrt_rn((3.0 * r) * r); }
 
 // Sequential mean: the block stages chunks of positions through LDS

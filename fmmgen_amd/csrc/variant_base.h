@@ -448,6 +448,10 @@ struct OrderTable {
     if (order == P) op_batch_order<P>(op, n, xyz, in, is, out, os, st);
     else OrderTable<P + 1>::op(order, op, n, xyz, in, is, out, os, st);
   }
+  static void bh(int order, Tree& t, double* F, hipStream_t st) {
+    if (order == P) bh_order<P>(t, F, st);
+    else OrderTable<P + 1>::bh(order, t, F, st);
+  }
 };
 template <>
 struct OrderTable<PMAX> {
@@ -459,6 +463,7 @@ struct OrderTable<PMAX> {
   static void op(int, int, int, const double*, const double*, int, double*, int, hipStream_t) {
     throw Error{kUnsupported, "order not compiled"};
   }
+  static void bh(int, Tree&, double*, hipStream_t) { throw Error{kUnsupported, "order not compiled"}; }
 };
 
 // Tree::compute_field_fmm (example/tree.cpp:243-271): zero F, M and L,
@@ -481,6 +486,24 @@ void evaluate(Tree& t, int mode, int mask, double* F, hipStream_t st) {
   if (mask & kPhaseP2P) near_field(t, mode, st);
   OrderTable<PMIN>::l2p(t.order, t, mode, mask, F, st);
   t.ml_order = mode == 0 ? t.order : -1;  // FP64 M / L hold this evaluation's expansions
+}
+
+// Tree::compute_field_bh (example/tree.cpp:273-293): F overwritten; the
+// multipoles by P2M and M2M as in the FMM, then every particle's walk
+// (FP64).
+void evaluate_bh(Tree& t, double* F, hipStream_t st) {
+  const int n = (int)t.n;
+  const int nc = t.ncells;
+  const size_t ms = (size_t)msize(t.order, SO);
+  t.M.alloc((size_t)nc * ms);
+  t.pS.alloc((size_t)std::max(n, 1) * SS);
+  {
+    PhaseTimer pt(t, kTPrep, st);
+    k_gather_S<<<nblk(n, 256), 256, 0, st>>>(n, t.perm.get(), t.user_S, t.pS.get());
+  }
+  OrderTable<PMIN>::far(t.order, t, 0, kPhaseP2M | kPhaseM2M, st);
+  OrderTable<PMIN>::bh(t.order, t, F, st);
+  t.ml_order = -1;  // L does not belong to this evaluation
 }
 
 // The caller's current source strengths in sorted order (t.pS).
@@ -720,7 +743,8 @@ const Variant FMM_VARIANT_NAME = {FMM_VARIANT_NS::SO,
                                   &FMM_VARIANT_NS::predict_cell_orders,
                                   &FMM_VARIANT_NS::gather_sources,
                                   &FMM_VARIANT_NS::op_batch,
-                                  &FMM_VARIANT_NS::direct};
+                                  &FMM_VARIANT_NS::direct,
+                                  &FMM_VARIANT_NS::evaluate_bh};
 #endif
 
 }  // namespace fmm

@@ -31,6 +31,8 @@ void l2p_final_order(Tree& t, int mode, int mask, double* F, hipStream_t st);
 template <int P>
 void op_batch_order(int op, int n, const double* xyz, const double* in, int in_stride, double* out,
                     int out_stride, hipStream_t st);
+template <int P>
+void bh_order(Tree& t, double* F, hipStream_t st);
 template <int Q>
 void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hipStream_t st);
 

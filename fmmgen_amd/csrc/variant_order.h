@@ -403,6 +403,129 @@ __global__ __launch_bounds__(64) void k_op_batch_dsum(int n, const double* __res
   }
 }
 
+// Barnes-Hut (Tree::compute_field_bh, example/tree.cpp:273-293, and
+// evaluate_M2P_and_P2P, example/calculate.cpp:266-314), FP64.  Every
+// particle walks the tree from the root: below an internal cell each child
+// c is opened when r_c > theta |x_i - centre_c| and applied by M2P at
+// x_i - centre_c otherwise; a leaf adds P2P from its particles except i.
+// One wave takes 64 consecutive sorted particles (one or two leaves
+// mostly) and walks the union of their paths on a stack in LDS whose
+// entries carry the lanes that reached the cell; a lane applies M2P only
+// at the cells where its own walk stops, so each particle sums exactly the
+// reference's terms.  The distance of the criterion is rounded as the
+// reference's host code rounds it, so every decision matches.
+constexpr int kBHStack = 256;  // a pop pushes at most 8: 7 per level + 1
+
+template <int P>
+__device__ __forceinline__ void bh_m2p(double dx, double dy, double dz, const double* __restrict__ Mc,
+                                       double* acc) {
+  if constexpr (P <= FMMGEN_FULL_BODIES_MAXORDER) {
+    fmmgen_ops::M2P_o<P, double>(dx, dy, dz, Mc, acc);
+  } else {
+    // the derivative block of order P + 1 (P without the field) and the
+    // sums of M2P over the graded layout
+    constexpr int MS = Op<P>::MS, OFF = nterms(SO - 1);
+    constexpr int PD = OUT != 1 ? P + 1 : P, DS = msize(PD, SO);
+    constexpr bool POT = OUT != 3;
+    double D[DS];
+    fmmgen_ops::Dgen_o<PD, double>(dx, dy, dz, D);
+    double phi = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int m = 0; m < MS; ++m) {
+      const MultiIdx b = slot_multi(m + OFF);
+      const double w = Mc[m];
+      if (POT) phi += w * D[m];
+      if (OUT != 1) {
+        fx += w * D[multi_slot(b.x + 1, b.y, b.z) - OFF];
+        fy += w * D[multi_slot(b.x, b.y + 1, b.z) - OFF];
+        fz += w * D[multi_slot(b.x, b.y, b.z + 1) - OFF];
+      }
+    }
+    int k = 0;
+    if (POT) acc[k++] += phi;
+    if (OUT != 1) {
+      acc[k] -= fx;
+      acc[k + 1] -= fy;
+      acc[k + 2] -= fz;
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(64) void k_bh(int n, const int* __restrict__ first_child, const int* __restrict__ nchild,
+                                           const double* __restrict__ cx, const double* __restrict__ cy,
+                                           const double* __restrict__ cz, const double* __restrict__ cr,
+                                           const int* __restrict__ pstart, const int* __restrict__ pcount,
+                                           const double* __restrict__ px, const double* __restrict__ py,
+                                           const double* __restrict__ pz, const double* __restrict__ pS,
+                                           const int* __restrict__ perm, const double* __restrict__ M, double theta,
+                                           double* __restrict__ F) {
+  constexpr int MS = Op<P>::MS;
+  __shared__ int s_cell[kBHStack];
+  __shared__ unsigned long long s_lanes[kBHStack];
+  const int lane = threadIdx.x;
+  const int s = blockIdx.x * 64 + lane;
+  const bool valid = s < n;
+  const double xi = valid ? px[s] : 0.0, yi = valid ? py[s] : 0.0, zi = valid ? pz[s] : 0.0;
+  double acc[OUT];
+#pragma unroll
+  for (int k = 0; k < OUT; ++k) acc[k] = 0.0;
+  const unsigned long long all = __ballot(valid);
+  if (lane == 0) {
+    s_cell[0] = 0;
+    s_lanes[0] = all;
+  }
+  int top = 1;  // uniform over the wave
+  __syncthreads();
+  while (top > 0) {
+    --top;
+    const int p = s_cell[top];
+    const bool mine = (s_lanes[top] >> lane) & 1ull;
+    __syncthreads();  // the entry is read before a push overwrites it
+    const int nch = nchild[p];
+    if (nch > 0) {
+      const int fc = first_child[p];
+      // children pushed last to first, so they are walked in octant order
+      for (int k = nch - 1; k >= 0; --k) {
+        const int c = fc + k;
+        const double dx = xi - cx[c], dy = yi - cy[c], dz = zi - cz[c];
+        const bool open = mine && cr[c] > theta * norm3_rn(dx, dy, dz);
+        if (mine && !open) bh_m2p<P>(dx, dy, dz, M + (size_t)c * MS, acc);
+        const unsigned long long lanes = __ballot(open);
+        if (lanes != 0ull && top < kBHStack) {
+          if (lane == 0) {
+            s_cell[top] = c;
+            s_lanes[top] = lanes;
+          }
+          ++top;
+        }
+      }
+      __syncthreads();
+    } else if (mine) {
+      const int b = pstart[p], e = b + pcount[p];
+      for (int j = b; j < e; ++j) {
+        if (j == s) continue;
+        fmmgen_ops::P2P<double>(xi - px[j], yi - py[j], zi - pz[j], pS + (size_t)j * SS, acc);
+      }
+    }
+  }
+  if (valid) {
+    const size_t i = (size_t)perm[s];
+#pragma unroll
+    for (int k = 0; k < OUT; ++k) F[i * OUT + k] = acc[k];
+  }
+}
+
+template <int P>
+void bh_order(Tree& t, double* F, hipStream_t st) {
+  const int n = (int)t.n;
+  FMM_CHECK(7 * t.depth + 1 <= kBHStack, kUnsupported, "tree too deep for the Barnes-Hut walk");
+  if (n == 0) return;
+  PhaseTimer pt(t, kTP2P, st);
+  k_bh<P><<<nblk(n, 64), 64, 0, st>>>(n, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(), t.cz.get(),
+                                       t.cr.get(), t.pstart.get(), t.pcount.get(), t.px.get(), t.py.get(),
+                                       t.pz.get(), t.pS.get(), t.perm.get(), t.M.get(), t.theta, F);
+}
+
 template <int P>
 void op_batch_order(int op, int n, const double* xyz, const double* in, int is, double* out, int os,
                     hipStream_t st) {
@@ -452,6 +575,7 @@ void op_batch_order(int op, int n, const double* xyz, const double* in, int is, 
   template void far_field_order<P>(Tree&, int, int, hipStream_t);                                       \
   template void l2p_final_order<P>(Tree&, int, int, double*, hipStream_t);                              \
   template void op_batch_order<P>(int, int, const double*, const double*, int, double*, int, hipStream_t); \
+  template void bh_order<P>(Tree&, double*, hipStream_t);                                                 \
   template void m2l_class<P>(Tree&, int, const int*, int, int, int, hipStream_t);                         \
   }                                                                                                     \
   }

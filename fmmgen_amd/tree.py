@@ -159,6 +159,16 @@ class Tree:
         _lib.check(_lib.load().fmm_evaluate_phases(self._h, m, int(phases), _ptr(F), _stream()))
         return F
 
+    def compute_field_bh(self, F):
+        """Tree::compute_field_bh (example/tree.cpp:273-293): Barnes-Hut in
+        FP64 at the tree's order and theta, into F [n, OUTPUTSIZE]
+        (overwritten)."""
+        _check_dev(F, "F", self.outputsize)
+        if F.shape[0] != self.n:
+            raise ValueError("F must have one row per particle")
+        _lib.check(_lib.load().fmm_evaluate_bh(self._h, _ptr(F), _stream()))
+        return F
+
     def compute_field_exact(self, F, targets=None):
         """Tree::compute_field_exact (example/tree.cpp:295-297): FP64 direct sum.
 

@@ -16,6 +16,9 @@
  *                         (example/tree.cpp:217-222)
  *   fmm_evaluate          Tree::compute_field_fmm(F)
  *                         example/tree.hpp:89, example/tree.cpp:243-271
+ *   fmm_evaluate_bh       Tree::compute_field_bh(F)
+ *                         example/tree.hpp:90, example/tree.cpp:273-293,
+ *                         example/calculate.cpp:266-314
  *   fmm_direct_sampled    Tree::compute_field_exact(F) on a target subset
  *                         example/tree.hpp:91, example/calculate.cpp:252-264
  *   fmm_op_batch          the generated operators P2M/M2M/M2L/L2L/L2P/M2P
@@ -79,7 +82,8 @@ int fmm_variant_orders(int source_order, int output_size, int* min_order, int* m
 /* Builds the octree and the M2L / P2P lists.  d_pos [n*3] interleaved x,y,z;
  * d_S [n*SOURCESIZE] is borrowed (read at every evaluation) and must stay
  * valid for the life of the tree, as build_tree keeps pointers into the
- * caller's arrays (example/tree.cpp:133-137). */
+ * caller's arrays (example/tree.cpp:133-137).  theta >= 0 (0: every pair of
+ * leaves interacts by P2P, as in the reference). */
 int fmm_tree_build(const double* d_pos, const double* d_S, size_t n, size_t ncrit, int order, double theta,
                    int source_order, int output_size, void* stream, fmm_tree_t* out);
 int fmm_tree_destroy(fmm_tree_t tree);
@@ -113,6 +117,14 @@ int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, 
 #define FMM_PHASE_TIMERS 10
 int fmm_timing_enable(fmm_tree_t tree, int enable);
 int fmm_timing_read(fmm_tree_t tree, double* h_ms, long long* h_count);
+
+/* Barnes-Hut evaluation in FP64 (Tree::compute_field_bh): multipoles by
+ * P2M and M2M at the tree's order, then per particle a walk from the root
+ * that opens a child cell when its half-width exceeds theta times the
+ * distance from the particle to its centre, applies M2P otherwise, and P2P
+ * in the leaves reached.  d_F [n*OUTPUTSIZE] in caller order, overwritten.
+ * Stream ordered. */
+int fmm_evaluate_bh(fmm_tree_t tree, double* d_F, void* stream);
 
 /* FP64 direct sum at d_targets[nt] over all n sources, d_F [nt*OUTPUTSIZE]. */
 int fmm_direct_sampled(const double* d_pos, const double* d_S, size_t n, const long long* d_targets, size_t nt,

@@ -62,6 +62,7 @@ def _load(path):
     L.oracle_export_ML.argtypes = [ctypes.c_void_p, c_double_p, c_double_p]
     L.oracle_compute_field_fmm.argtypes = [ctypes.c_void_p, c_double_p]
     L.oracle_compute_field_fmm_phases.argtypes = [ctypes.c_void_p, c_double_p, ctypes.c_int]
+    L.oracle_compute_field_bh.argtypes = [ctypes.c_void_p, c_double_p]
     L.oracle_direct_targets.argtypes = [c_double_p, c_double_p, ctypes.c_size_t, c_int64_p,
                                         ctypes.c_size_t, c_double_p]
     for nm in ("oracle_op_P2M",):
@@ -123,6 +124,11 @@ class OracleTree:
     def compute_field_fmm(self, mask=63):
         F = np.zeros((self.n, self.outputsize))
         self.lib.oracle_compute_field_fmm_phases(self.h, _p(F), int(mask))
+        return F
+
+    def compute_field_bh(self):
+        F = np.zeros((self.n, self.outputsize))
+        self.lib.oracle_compute_field_bh(self.h, _p(F))
         return F
 
     def ML(self, msize, lsize):

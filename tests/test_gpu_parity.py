@@ -252,6 +252,47 @@ def test_fmm_mixed_matches_oracle(gpu, kind, n, ncrit, order, theta, so, mode):
     assert util.rel_norm(Fm.cpu().numpy()[:, fo:], F64.cpu().numpy()[:, fo:]) < 1e-6
 
 
+# ------------------------------------------------------------------ Barnes-Hut
+
+BH_CASES = [
+    # kind, n, ncrit, order, theta, so
+    ("uniform", 20000, 64, 4, 0.5, 0),
+    ("plummer", 20000, 32, 8, 0.4, 0),
+    ("blobs", 20000, 64, 10, 0.3, 0),
+    ("blobs", 10000, 64, 12, 0.5, 0),
+    ("mixed", 20000, 128, 9, 0.5, 0),
+    ("plummer", 20000, 32, 6, 0.4, 1),
+    ("uniform", 3000, 16, 5, 0.0, 0),     # every cell opened: the direct sum
+    ("uniform", 50, 64, 5, 0.5, 0),       # root is a leaf
+    ("uniform", 1, 64, 3, 0.5, 0),
+]
+
+
+@pytest.mark.parametrize("kind,n,ncrit,order,theta,so", BH_CASES)
+def test_bh_matches_oracle(gpu, kind, n, ncrit, order, theta, so):
+    """Tree::compute_field_bh against the oracle's restatement: every
+    opening decision is taken as the reference takes it, so the sums agree
+    to FP64 rounding."""
+    from fmmgen_amd import build_tree
+    _log(f"bh {kind} n={n} ncrit={ncrit} p={order} theta={theta} so={so}")
+    pos = _positions(kind, n, 17)
+    S = np.random.default_rng(18).uniform(-1, 1, (n, 1)) if so == 0 else util.dipoles(n, 18)
+    outsz = 4 if so == 0 else 3
+    ot = O.OracleTree(pos, S, ncrit, order, theta, source_order=so, outputsize=outsz)
+    Fo = ot.compute_field_bh()
+    gt = build_tree(_dev(pos), _dev(S), n, ncrit, order, theta, source_order=so, potential=(so == 0))
+    F = torch.full((n, outsz), float("nan"), dtype=torch.float64, device="cuda")
+    gt.compute_field_bh(F)
+    Fg = F.cpu().numpy()
+    if n == 1:
+        assert np.all(Fg == 0.0)
+        return
+    assert util.rel_norm(Fg, Fo) < TOL
+    # an FMM evaluation after it still matches (the BH pass leaves no state behind)
+    gt.compute_field_fmm(F, mode="plain")
+    assert util.rel_norm(F.cpu().numpy(), ot.compute_field_fmm()) < TOL
+
+
 # ------------------------------------------------------------------ tree reuse
 
 def test_tree_reuse_with_new_strengths_matches_oracle(gpu):
