@@ -24,7 +24,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # header and the name of the variant table in the library.
 VARIANTS = {
     "so0_pf": dict(order=13, source_order=0, potential=True, field=True),
+    "so0_p": dict(order=13, source_order=0, potential=True, field=False),
+    "so0_f": dict(order=13, source_order=0, potential=False, field=True),
     "so1_f": dict(order=9, source_order=1, potential=False, field=True),
+    "so2_pf": dict(order=9, source_order=2, potential=True, field=True),
 }
 
 HIPFLAGS = [

@@ -63,8 +63,8 @@ void timing_collect(Tree& t) {
 const char* last_error() { return g_err.c_str(); }
 
 const Variant* find_variant(int so, int outsz) {
-  if (so == variant_so0_pf.so && outsz == variant_so0_pf.outsz) return &variant_so0_pf;
-  if (so == variant_so1_f.so && outsz == variant_so1_f.outsz) return &variant_so1_f;
+  for (const Variant* v : {&variant_so0_pf, &variant_so0_p, &variant_so0_f, &variant_so1_f, &variant_so2_pf})
+    if (so == v->so && outsz == v->outsz) return v;
   return nullptr;
 }
 
@@ -215,6 +215,7 @@ int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, 
     if (mode == FMM_MODE_LEARNED_F32 || mode == FMM_MODE_LEARNED_BF16X3) {
       FMM_CHECK(phase_mask == kPhaseAll, kInvalidArgument, "the learned modes run every phase");
       FMM_CHECK(tree->t.so == 0, kUnsupported, "the learned modes need source order 0");
+      FMM_CHECK(tree->t.outsz != 1, kUnsupported, "the learned modes need the field (their error model)");
       tree->v->evaluate_learned(tree->t, mode, d_F, static_cast<hipStream_t>(stream));
       FMM_HIP(hipGetLastError());
       return;
@@ -459,6 +460,8 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
   return guarded([&] {
     FMM_CHECK(tree && d_params, kInvalidArgument, "NULL argument");
     FMM_CHECK(tol > 0 && tol < 1, kInvalidArgument, "tolerance must be in (0, 1)");
+    FMM_CHECK(tree->t.so == 0 && tree->t.outsz != 1, kUnsupported,
+              "the order model needs charges and the field (its error model is the field's)");
     Tree& t = tree->t;
     // no evaluation on any stream may still read the buffers re-formed here
     FMM_HIP(hipDeviceSynchronize());

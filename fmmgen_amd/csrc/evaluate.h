@@ -43,7 +43,10 @@ struct Variant {
 };
 
 extern const Variant variant_so0_pf;  // charges, potential + field
+extern const Variant variant_so0_p;   // charges, potential only
+extern const Variant variant_so0_f;   // charges, field only
 extern const Variant variant_so1_f;   // dipoles, field only
+extern const Variant variant_so2_pf;  // quadrupoles, potential + field
 
 const Variant* find_variant(int so, int outsz);
 

@@ -14,7 +14,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "_build")
 
-_VARIANTS = {(0, 4): "liboracle_so0_pf.so", (1, 3): "liboracle_so1_f.so"}
+_VARIANTS = {(0, 4): "liboracle_so0_pf.so", (0, 1): "liboracle_so0_p.so", (0, 3): "liboracle_so0_f.so",
+             (1, 3): "liboracle_so1_f.so", (2, 4): "liboracle_so2_pf.so"}
 _libs = {}
 
 c_double_p = ctypes.POINTER(ctypes.c_double)

@@ -189,13 +189,29 @@ def M2P(x, M, p, so, potential=True, field=True):
 
 
 def P2P(x, S, so, potential=True, field=True):
-    """so=0: phi = q/R, F = q x/R^3; so=1: phi = S . grad(1/R), F = -grad phi
-    (fmmgen/generator.py:257-288)."""
+    """so=0: phi = q/R, F = q x/R^3; so=1: phi = S . grad(1/R); so=2:
+    phi = sum_s S_s d^s (1/R); F = -grad phi (fmmgen/generator.py:257-288)."""
     X = [mp.mpf(v) for v in x]
     R = mp.sqrt(X[0] ** 2 + X[1] ** 2 + X[2] ** 2)
     if so == 0:
         q = mp.mpf(S[0])
         return _fields(q / R, [q * X[a] / R ** 3 for a in range(3)], potential, field)
+    if so == 2:
+        # quadrupole source S_s, |s| = 2, s = e_a + e_b in graded slot order:
+        # phi = sum_s S_s d_a d_b (1/R),  d_a d_b (1/R) = 3 x_a x_b/R^5 - delta_ab/R^3,
+        # d_c d_a d_b (1/R) = 3 (delta_ac x_b + delta_bc x_a + delta_ab x_c)/R^5 - 15 x_a x_b x_c/R^7
+        pairs = []
+        for n in monomials(2, 2):
+            ab = [a for a in range(3) for _ in range(n[a])]
+            pairs.append((ab[0], ab[1]))
+        d = lambda i, j: 1 if i == j else 0  # noqa: E731
+        phi = sum(mp.mpf(S[i]) * (3 * X[a] * X[b] / R ** 5 - d(a, b) / R ** 3) for i, (a, b) in enumerate(pairs))
+        grads = []
+        for c in range(3):
+            g = sum(mp.mpf(S[i]) * (3 * (d(a, c) * X[b] + d(b, c) * X[a] + d(a, b) * X[c]) / R ** 5
+                                    - 15 * X[a] * X[b] * X[c] / R ** 7) for i, (a, b) in enumerate(pairs))
+            grads.append(-g)
+        return _fields(phi, grads, potential, field)
     # dipole: grad(1/R) = -x/R^3, d_a d_b (1/R) = 3 x_a x_b / R^5 - delta_ab / R^3
     phi = -sum(mp.mpf(S[i]) * X[i] for i in range(3)) / R ** 3
     grads = []

@@ -305,6 +305,38 @@ def test_learned_full_size_blobs_meets_tolerance(gpu):
 
 
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
+def test_learned_field_only_variant(gpu):
+    """The learned modes on the field-only charge variant meet the
+    tolerance; the potential-only variant refuses them (the error model is
+    the field's)."""
+    from fmmgen_amd import build_tree, direct_sum
+    from fmmgen_amd import learned as LN
+    from fmmgen_amd import _lib
+    tol = 1e-6
+    n = 1 << 16
+    pos = util.blobs(n, 81)
+    q = np.random.default_rng(82).uniform(-1, 1, (n, 1))
+    dpos, dq = _dev(pos), _dev(q)
+    params, margin, _ = LN.load(MODEL)
+    t = build_tree(dpos, dq, n, 64, 12, 0.5, potential=False, field=True)
+    t.set_model(params, tol, margin)
+    F = torch.zeros(n, 3, dtype=torch.float64, device="cuda")
+    tg = _all_targets(n)
+    Fd = direct_sum(dpos, dq, torch.from_numpy(tg).cuda()).cpu().numpy()
+    for mode in ("learned", "learned_bf16x3"):
+        t.compute_field_fmm(F, mode=mode)
+        Fx = np.concatenate([np.zeros((n, 1)), F.cpu().numpy()], 1)
+        e, et = _errors(Fx, Fd)
+        _log(f"  field-only {mode}: rel field error {e:.3e}, trimmed {et:.3e}")
+        assert e <= tol
+    tp = build_tree(dpos, dq, n, 64, 12, 0.5, potential=True, field=False)
+    with pytest.raises(_lib.FmmError):
+        tp.set_model(params, tol, margin)
+    with pytest.raises(_lib.FmmError):
+        tp.compute_field_fmm(torch.zeros(n, 1, dtype=torch.float64, device="cuda"), mode="learned")
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
 def test_learned_tree_reuse_after_new_strengths(gpu):
     """New strengths on a tree with a model (fmm_tree_set_sources): the
     learned evaluation meets the tolerance with the old tiles, and again

@@ -35,7 +35,7 @@ def _log(msg):
 
 # ------------------------------------------------------------------ operators
 
-@pytest.mark.parametrize("so,outsz", [(0, 4), (1, 3)])
+@pytest.mark.parametrize("so,outsz", [(0, 4), (1, 3), (0, 1), (0, 3), (2, 4)])
 def test_operator_batches_match_oracle(gpu, so, outsz):
     """Every operator at every compiled order (whole generated bodies up to
     the full-body order; chunked bodies and the derivative-block sums
@@ -250,6 +250,61 @@ def test_fmm_mixed_matches_oracle(gpu, kind, n, ncrit, order, theta, so, mode):
     gt.compute_field_fmm(F64, mode="plain")
     assert util.rel_norm(F64.cpu().numpy(), Fo) < TOL
     assert util.rel_norm(Fm.cpu().numpy()[:, fo:], F64.cpu().numpy()[:, fo:]) < 1e-6
+
+
+# ------------------------------------------------------------------ other variants
+
+VARIANT_CASES = [
+    # kind, n, ncrit, order, theta, source order, output size
+    ("uniform", 20000, 64, 0, 0.5, 0, 1),     # potential only, monopoles only
+    ("plummer", 20000, 32, 7, 0.4, 0, 1),
+    ("blobs", 20000, 64, 10, 0.5, 0, 3),      # field only
+    ("uniform", 20000, 64, 4, 0.5, 2, 4),     # quadrupole sources
+    ("plummer", 20000, 32, 8, 0.4, 2, 4),
+]
+
+
+def _sources(so, n, seed):
+    from fmmgen_amd.codegen.layout import sourcesize
+    return np.random.default_rng(seed).uniform(-1, 1, (n, sourcesize(so)))
+
+
+@pytest.mark.parametrize("kind,n,ncrit,order,theta,so,outsz", VARIANT_CASES)
+def test_other_variants_match_oracle(gpu, kind, n, ncrit, order, theta, so, outsz):
+    """Potential-only and field-only charge variants and quadrupole sources:
+    plain FMM and Barnes-Hut <= 1e-12 against the oracle's variant; the
+    mixed modes <= 1e-6 on the field components, and on the potential
+    1e-6 (FP32 M2L) or 1e-5 (BF16x3 M2L: about 16 significand bits per
+    product, and the potential, unlike the field, is mostly far field)."""
+    from fmmgen_amd import build_tree
+    _log(f"variant so={so} outsz={outsz} {kind} n={n} p={order}")
+    pos = _positions(kind, n, 23)
+    S = _sources(so, n, 24)
+    ot = O.OracleTree(pos, S, ncrit, order, theta, source_order=so, outputsize=outsz)
+    gt = build_tree(_dev(pos), _dev(S), n, ncrit, order, theta, source_order=so,
+                    potential=outsz in (1, 4), field=outsz in (3, 4))
+    F = torch.zeros(n, outsz, dtype=torch.float64, device="cuda")
+    Fo = ot.compute_field_fmm()
+    gt.compute_field_fmm(F, mode="plain")
+    assert util.rel_norm(F.cpu().numpy(), Fo) < TOL
+    fo = 1 if outsz == 4 else 0      # field components
+    for mode in ("f32", "bf16x3"):
+        gt.compute_field_fmm(F, mode=mode)
+        Fm = F.cpu().numpy()
+        if outsz == 1:
+            err = util.rel_norm(Fm, Fo)
+            _log(f"  {mode} potential: {err:.3e}")
+            assert err < (1e-6 if mode == "f32" else 1e-5), mode
+        else:
+            err = util.rel_norm(Fm[:, fo:], Fo[:, fo:])
+            _log(f"  {mode} field: {err:.3e}")
+            assert err < 1e-6, mode
+            if outsz == 4:
+                ep = util.rel_norm(Fm[:, 0], Fo[:, 0])
+                _log(f"  {mode} potential: {ep:.3e}")
+                assert ep < (1e-6 if mode == "f32" else 1e-5), mode
+    gt.compute_field_bh(F)
+    assert util.rel_norm(F.cpu().numpy(), ot.compute_field_bh()) < TOL
 
 
 # ------------------------------------------------------------------ Barnes-Hut

@@ -27,12 +27,17 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-# every compiled order of the two operator variants (oracle and HIP library)
-VARIANTS = [(0, 4, list(range(1, 13))), (1, 3, list(range(2, 9)))]
+# every compiled order of the operator variants with their own operator
+# programs (oracle and HIP library): (source order, output size, potential,
+# field, orders).  The charge variants with potential only or field only
+# share every program but L2P / M2P / P2P with the first; they are pinned
+# against it below.
+VARIANTS = [(0, 4, True, True, list(range(1, 13))), (1, 3, False, True, list(range(2, 9))),
+            (2, 4, True, True, list(range(3, 9)))]
 
 
-@pytest.mark.parametrize("so,outsz,orders", VARIANTS)
-def test_shift_and_p2m_operators(so, outsz, orders):
+@pytest.mark.parametrize("so,outsz,pot,field,orders", VARIANTS)
+def test_shift_and_p2m_operators(so, outsz, pot, field, orders):
     lib = O.lib(so, outsz)
     rng = np.random.default_rng(1 + so)
     for p in orders:
@@ -42,15 +47,15 @@ def test_shift_and_p2m_operators(so, outsz, orders):
         L = rng.uniform(-1, 1, ls)
         assert _rel(_call(lib.oracle_op_M2M, x, M, ms, p), CF.M2M(x, M, p, so)) < TOL, ("M2M", p)
         assert _rel(_call(lib.oracle_op_L2L, x, L, ls, p), CF.L2L(x, L, p, so)) < TOL, ("L2L", p)
-        assert _rel(_call(lib.oracle_op_L2P, x, L, outsz, p), CF.L2P(x, L, p, so, so == 0, True)) < TOL, ("L2P", p)
+        assert _rel(_call(lib.oracle_op_L2P, x, L, outsz, p), CF.L2P(x, L, p, so, pot, field)) < TOL, ("L2P", p)
         # the reference P2M is the charge formula in the so-offset slots
         o = np.zeros(ms)
         lib.oracle_op_P2M(x[0], x[1], x[2], 0.7, o.ctypes.data_as(O.c_double_p), p)
         assert _rel(o, CF.P2M(x, 0.7, p, so)) < TOL, ("P2M", p)
 
 
-@pytest.mark.parametrize("so,outsz,orders", VARIANTS)
-def test_translation_operators(so, outsz, orders):
+@pytest.mark.parametrize("so,outsz,pot,field,orders", VARIANTS)
+def test_translation_operators(so, outsz, pot, field, orders):
     lib = O.lib(so, outsz)
     rng = np.random.default_rng(11 + so)
     for p in orders:
@@ -59,17 +64,47 @@ def test_translation_operators(so, outsz, orders):
         M = rng.uniform(-1, 1, ms)
         assert _rel(_call(lib.oracle_op_M2L, x, M, ls, p), CF.M2L(tuple(x), M, p, so)) < 1e-13, ("M2L", p)
         assert _rel(_call(lib.oracle_op_M2P, x, M, outsz, p),
-                    CF.M2P(tuple(x), M, p, so, so == 0, True)) < 1e-13, ("M2P", p)
+                    CF.M2P(tuple(x), M, p, so, pot, field)) < 1e-13, ("M2P", p)
 
 
-@pytest.mark.parametrize("so,outsz", [(0, 4), (1, 3)])
-def test_p2p(so, outsz):
+@pytest.mark.parametrize("so,outsz,pot,field", [(0, 4, True, True), (1, 3, False, True), (2, 4, True, True),
+                                                (0, 1, True, False), (0, 3, False, True)])
+def test_p2p(so, outsz, pot, field):
+    from fmmgen_amd.codegen.layout import sourcesize
     lib = O.lib(so, outsz)
     rng = np.random.default_rng(5)
     for _ in range(20):
         x = rng.uniform(-1, 1, 3)
-        S = rng.uniform(-1, 1, 1 if so == 0 else 3)
-        assert _rel(_call(lib.oracle_op_P2P, x, S, outsz), CF.P2P(x, S, so, so == 0, True)) < TOL
+        S = rng.uniform(-1, 1, sourcesize(so))
+        assert _rel(_call(lib.oracle_op_P2P, x, S, outsz), CF.P2P(x, S, so, pot, field)) < TOL
+
+
+@pytest.mark.parametrize("outsz,keep,orders", [(1, [0], list(range(0, 13))), (3, [1, 2, 3], list(range(1, 13)))])
+def test_output_variants_agree_with_potential_and_field(outsz, keep, orders):
+    """The charge variants with potential only / field only: the expansion
+    operators are the pinned variant's exactly, and L2P / M2P give its
+    components; order 0 (potential only) against the closed forms."""
+    lib, ref = O.lib(0, outsz), O.lib(0, 4)
+    rng = np.random.default_rng(21)
+    for p in orders:
+        ms, ls = Msize(p, 0), Lsize(p, 0)
+        x = rng.uniform(-0.6, 0.6, 3)
+        far = rng.uniform(1.0, 2.0, 3) * rng.choice([-1, 1], 3)
+        M = rng.uniform(-1, 1, ms)
+        L = rng.uniform(-1, 1, ls)
+        if p == 0:
+            assert _rel(_call(lib.oracle_op_L2P, x, L, 1, 0), CF.L2P(x, L, 0, 0, True, False)) < TOL
+            assert _rel(_call(lib.oracle_op_M2P, far, M, 1, 0), CF.M2P(tuple(far), M, 0, 0, True, False)) < TOL
+            assert _rel(_call(lib.oracle_op_M2L, far, M, 1, 0), CF.M2L(tuple(far), M, 0, 0)) < TOL
+            continue
+        for op, xx, inp, n in (("M2M", x, M, ms), ("L2L", x, L, ls), ("M2L", far, M, ls)):
+            a = _call(getattr(lib, f"oracle_op_{op}"), xx, inp, n, p)
+            b = _call(getattr(ref, f"oracle_op_{op}"), xx, inp, n, p)
+            assert np.array_equal(a, b), (op, p)
+        for op, xx, inp in (("L2P", x, L), ("M2P", far, M)):
+            a = _call(getattr(lib, f"oracle_op_{op}"), xx, inp, outsz, p)
+            b = _call(getattr(ref, f"oracle_op_{op}"), xx, inp, 4, p)[keep]
+            assert np.allclose(a, b, rtol=1e-15, atol=0), (op, p)
 
 
 def test_shift_composition_is_exact():

@@ -213,3 +213,20 @@ def test_fmm_error_falls_with_order_and_charge_is_conserved():
         assert abs(M[0, 0] - q.sum()) <= 1e-10 * np.abs(q).sum()
     assert all(errs[i + 1] < errs[i] / 3 for i in range(len(errs) - 1))
     assert errs[-1] < 1e-4
+
+
+@pytest.mark.parametrize("so,outsz,orders", [(1, 3, (2, 4, 6, 8)), (2, 4, (3, 5, 7)), (0, 1, (0, 2, 4, 6)),
+                                             (0, 3, (1, 3, 5, 7))])
+def test_fmm_variants_converge_to_direct_sum(so, outsz, orders):
+    """The oracle FMM of the other operator variants (dipole and quadrupole
+    sources, potential only, field only) approaches the direct sum as the
+    order grows."""
+    from fmmgen_amd.codegen.layout import sourcesize
+    pos, _ = util.uniform(3000, 19)
+    S = np.random.default_rng(20).uniform(-1, 1, (3000, sourcesize(so)))
+    tg = np.arange(0, 3000, 11)
+    Fd = O.direct_targets(pos, S, tg, source_order=so, outputsize=outsz)
+    errs = [util.rel_norm(O.OracleTree(pos, S, 32, p, 0.5, source_order=so, outputsize=outsz)
+                          .compute_field_fmm()[tg], Fd) for p in orders]
+    assert all(b < a / 3 for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < 1e-3, errs
