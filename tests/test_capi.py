@@ -34,7 +34,11 @@ def test_argument_errors_are_reported():
     assert mn.value == 1 and mx.value > 8
     assert L.fmm_variant_orders(1, 3, ctypes.byref(mn), ctypes.byref(mx)) == _lib.FMM_OK
     assert mn.value == 2
-    assert L.fmm_variant_orders(2, 4, ctypes.byref(mn), ctypes.byref(mx)) == 2
+    assert L.fmm_variant_orders(2, 4, ctypes.byref(mn), ctypes.byref(mx)) == _lib.FMM_OK
+    assert (mn.value, mx.value) == (3, 9)
+    assert L.fmm_variant_orders(0, 1, ctypes.byref(mn), ctypes.byref(mx)) == _lib.FMM_OK
+    assert mn.value == 0
+    assert L.fmm_variant_orders(3, 4, ctypes.byref(mn), ctypes.byref(mx)) == 2
     assert b"no compiled variant" in L.fmm_last_error()
     h = ctypes.c_void_p()
     assert L.fmm_tree_build(None, None, 10, 64, 4, 0.5, 0, 4, None, ctypes.byref(h)) == 1
