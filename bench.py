@@ -182,6 +182,7 @@ def main():
                     choices=["headline", "plain", "f32", "bf16x3", "learned", "learned_bf16x3"])
     ap.add_argument("--order", type=int, default=None)
     ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--theta", type=float, default=None)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--model", default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -195,6 +196,8 @@ def main():
     cfg = dict(CONFIGS[args.config])
     if args.order:
         cfg["order"] = args.order
+    if args.theta is not None:
+        cfg["theta"] = args.theta
     if args.n:
         cfg["n"] = args.n
     n, so = cfg["n"], cfg["so"]

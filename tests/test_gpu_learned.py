@@ -305,6 +305,34 @@ def test_learned_full_size_blobs_meets_tolerance(gpu):
 
 
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
+@pytest.mark.parametrize("theta", [0.3, 0.4, 0.55])
+def test_learned_other_opening_angles(gpu, theta):
+    """The network is trained over opening angles 0.2 .. 0.6 (the example
+    sweeps 0.1 .. 0.5): at other angles than the headline's the learned
+    evaluation meets the tolerance without falling back to the full order
+    everywhere."""
+    from fmmgen_amd import build_tree, direct_sum
+    from fmmgen_amd import learned as LN
+    tol = 1e-6
+    n = 1 << 17
+    pos = util.blobs(n, 91)
+    q = np.random.default_rng(92).uniform(-1, 1, (n, 1))
+    dpos, dq = _dev(pos), _dev(q)
+    t = build_tree(dpos, dq, n, 64, 12, theta)
+    params, margin, _ = LN.load(MODEL)
+    t.set_model(params, tol, margin)
+    F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+    t.compute_field_fmm(F, mode="learned")
+    st = t.learned_stats()
+    tg = _all_targets(n)
+    Fd = direct_sum(dpos, dq, torch.from_numpy(tg).cuda()).cpu().numpy()
+    e, et = _errors(F.cpu().numpy(), Fd)
+    _log(f"  theta {theta}: rel field error {e:.3e}, trimmed {et:.3e}, stats {st}")
+    assert e <= tol
+    assert st["m2l_mac_fraction"] < 0.6
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
 def test_learned_field_only_variant(gpu):
     """The learned modes on the field-only charge variant meet the
     tolerance; the potential-only variant refuses them (the error model is
