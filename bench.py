@@ -96,8 +96,14 @@ def timed(tree, F, mode, steps, warmup):
     evaluation on the evaluation stream, mean wall-clock ms per evaluation
     of the synchronised run, per-phase ms per evaluation)."""
     import torch
+
+    def run():
+        if mode == "bh":
+            tree.compute_field_bh(F)
+        else:
+            tree.compute_field_fmm(F, mode=mode)
     for _ in range(warmup):
-        tree.compute_field_fmm(F, mode=mode)
+        run()
     torch.cuda.synchronize()
     tree.timing(True)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
@@ -105,7 +111,7 @@ def timed(tree, F, mode, steps, warmup):
     t0 = time.perf_counter()
     ev[0].record()
     for k in range(steps):
-        tree.compute_field_fmm(F, mode=mode)
+        run()
         ev[k + 1].record()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -142,6 +148,10 @@ def error_detail(F, Fref, tg, so):
 def roofline_of(phases, info, order, so, mode, m2l_macs=None, config="c4"):
     """Roofline entry of the dominant kernel (P2P: VALU, M2L: MFMA); times are
     hipEvent phase times on the evaluation stream."""
+    if mode == "bh":  # the walk: per particle M2P and P2P, FP64 VALU
+        return {"bound": "valu", "kernel": "BH", "achieved": None, "peak": PEAK["fp64_vector_TF"],
+                "unit": "TFLOP/s", "frac": None, "traffic": None, "avg_ms": phases["bh"],
+                "flop_per_launch": None}
     dom = max(("P2P", "M2L"), key=lambda k: phases[k])
     if dom == "P2P":
         flops = P2P_FLOP_PER_PAIR * info["p2p_particle_pairs"]
@@ -179,7 +189,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--mode", default="headline",
-                    choices=["headline", "plain", "f32", "bf16x3", "learned", "learned_bf16x3"])
+                    choices=["headline", "plain", "f32", "bf16x3", "learned", "learned_bf16x3", "bh"])
     ap.add_argument("--order", type=int, default=None)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--theta", type=float, default=None)
@@ -271,7 +281,7 @@ def main():
 
     value = n / (ms_per_step * 1e-3)
     dtype = {"plain": "f64", "f32": "f32", "bf16x3": "bf16x3+f32", "learned": "f32",
-             "learned_bf16x3": "bf16x3+f32"}[mode]
+             "learned_bf16x3": "bf16x3+f32", "bh": "f64"}[mode]
     workload = (f"{args.config}: N={n} {cfg['dist']}, theta {cfg['theta']}, ncrit {cfg['ncrit']}, "
                 f"source_order {so}, " + (f"learned order 2-{full}, tol {args.tol:g}" if learned
                                           else f"order {cfg['order']}"))

@@ -38,7 +38,7 @@ EXPORTS = (
     "fmm_tree_set_order", "fmm_tree_leaf_features", "fmm_tree_leaf_error", "fmm_tree_leaves",
     "fmm_tree_set_model", "fmm_tree_learned_stats", "fmm_tree_leaf_orders", "fmm_tree_field_scale",
 )
-PHASE_NAMES = ("prep", "P2M", "M2M", "M2L", "P2P", "L2L", "L2P", "order", "judge", "unused")
+PHASE_NAMES = ("prep", "P2M", "M2M", "M2L", "P2P", "L2L", "L2P", "order", "judge", "bh")
 
 
 class FmmError(RuntimeError):

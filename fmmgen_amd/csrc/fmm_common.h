@@ -92,6 +92,7 @@ enum PhaseTimerId : int {
   kTL2P = 6,    // L2P plus the final per-target sum
   kTOrder = 7,  // learned order: features, network inference, cell and tile orders
   kTJudge = 8,  // learned order: sampled direct-sum judge and reruns
+  kTBH = 9,     // Barnes-Hut walk (M2P and P2P per particle)
   kNumPhaseTimers = 10,
 };
 

@@ -283,25 +283,25 @@ __global__ void k_pair_keys(long long m, const int2* __restrict__ pr, unsigned l
 }
 
 __global__ void k_csr_fill(long long m, const unsigned long long* __restrict__ key, int* __restrict__ src,
-                           int* __restrict__ row_first, int ncells) {
+                           int* __restrict__ row_first, int* __restrict__ row_end) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= m) return;
   const unsigned long long k = key[i];
   const int t = (int)(k >> 32);
   src[i] = (int)(k & 0xffffffffu);
   if (i == 0 || (int)(key[i - 1] >> 32) != t) row_first[t] = (int)i;
+  if (i == m - 1 || (int)(key[i + 1] >> 32) != t) row_end[t] = (int)(i + 1);
 }
 
-// Offsets of rows with no entry: the next non-empty row's start (or m).
-__global__ void k_csr_close(int ncells, int* __restrict__ off, long long m) {
-  // single pass from the back on one lane keeps it simple; ncells is small
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int next = (int)m;
-  off[ncells] = (int)m;
-  for (int c = ncells - 1; c >= 0; --c) {
-    if (off[c] < 0) off[c] = next;
-    else next = off[c];
-  }
+// Entries per row (0 for rows without any: both bounds stay zero).
+__global__ void k_csr_count(int ncells, const int* __restrict__ row_first, const int* __restrict__ row_end,
+                            int* __restrict__ cnt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) cnt[c] = row_end[c] - row_first[c];
+}
+
+__global__ void k_set_int(int* __restrict__ p, int v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
 }
 
 inline int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
@@ -329,11 +329,28 @@ void exclusive_scan_int(Scratch& s, const T* in, T* out, long long n, hipStream_
   FMM_HIP(rocprim::exclusive_scan(tmp, bytes, in, out, T(0), (size_t)n, rocprim::plus<T>(), st));
 }
 
+// CSR offsets off[0..nc] from each row's first and one-past-last entry
+// (both zero for a row without entries): the exclusive scan of the row
+// lengths, so an empty row's offset is the next row's start; off[nc] = m.
+void close_csr(Tree& t, const int* first, const int* end, int* off, int nc, long long m, hipStream_t st) {
+  DevBuf<int> cnt;
+  cnt.alloc(nc);
+  k_csr_count<<<blocks(nc, 256), 256, 0, st>>>(nc, first, end, cnt.get());
+  exclusive_scan_int(t.scratch, cnt.get(), off, nc, st);
+  k_set_int<<<1, 64, 0, st>>>(off + nc, (int)m);
+  FMM_HIP(hipStreamSynchronize(st));  // cnt is freed on return
+}
+
 void build_csr(Tree& t, DevBuf<int2>& pairs, long long m, DevBuf<int>& off, DevBuf<int>& src,
                hipStream_t st) {
-  off.alloc(t.ncells + 1);
+  const int nc = t.ncells;
+  off.alloc(nc + 1);
   src.alloc(std::max<long long>(m, 1));
-  FMM_HIP(hipMemsetAsync(off.get(), 0xff, sizeof(int) * (t.ncells + 1), st));
+  DevBuf<int> first, end;
+  first.alloc(nc);
+  end.alloc(nc);
+  FMM_HIP(hipMemsetAsync(first.get(), 0, sizeof(int) * nc, st));
+  FMM_HIP(hipMemsetAsync(end.get(), 0, sizeof(int) * nc, st));
   if (m > 0) {
     DevBuf<unsigned long long> k0, k1;
     k0.alloc(m);
@@ -343,10 +360,10 @@ void build_csr(Tree& t, DevBuf<int2>& pairs, long long m, DevBuf<int>& off, DevB
     FMM_HIP(rocprim::radix_sort_keys(nullptr, bytes, k0.get(), k1.get(), (size_t)m, 0, 64, st));
     void* tmp = t.scratch.get(bytes);
     FMM_HIP(rocprim::radix_sort_keys(tmp, bytes, k0.get(), k1.get(), (size_t)m, 0, 64, st));
-    k_csr_fill<<<blocks(m, 256), 256, 0, st>>>(m, k1.get(), src.get(), off.get(), t.ncells);
+    k_csr_fill<<<blocks(m, 256), 256, 0, st>>>(m, k1.get(), src.get(), first.get(), end.get());
     FMM_HIP(hipStreamSynchronize(st));
   }
-  k_csr_close<<<1, 1, 0, st>>>(t.ncells, off.get(), m);
+  close_csr(t, first.get(), end.get(), off.get(), nc, m, st);
 }
 
 // Source ranges: one entry per (target, non-self source) pair keyed by
@@ -390,7 +407,8 @@ __global__ void k_rng_flags(long long m, const unsigned long long* __restrict__ 
 
 __global__ void k_rng_fill(long long m, const unsigned long long* __restrict__ key, const int* __restrict__ srcB,
                            const int* __restrict__ pcount, const int* __restrict__ flag, const int* __restrict__ pos,
-                           int* __restrict__ rstart, int* __restrict__ rlen, int* __restrict__ row_first) {
+                           int* __restrict__ rstart, int* __restrict__ rlen, int* __restrict__ row_first,
+                           int* __restrict__ row_end) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= m) return;
   const int r = pos[i] + flag[i] - 1;
@@ -399,6 +417,7 @@ __global__ void k_rng_fill(long long m, const unsigned long long* __restrict__ k
     rstart[r] = (int)(key[i] & 0xffffffffu);
     if (i == 0 || (int)(key[i - 1] >> 32) != A) row_first[A] = r;
   }
+  if (i == m - 1 || (int)(key[i + 1] >> 32) != A) row_end[A] = r + 1;
   atomicAdd(&rlen[r], pcount[srcB[i]]);
 }
 
@@ -507,12 +526,11 @@ void build_p2p_ranges(Tree& t, hipStream_t st) {
   FMM_HIP(hipStreamSynchronize(st));
   const long long m = (long long)a + b;
   t.rng_off.alloc(nc + 1);
-  FMM_HIP(hipMemsetAsync(t.rng_off.get(), 0xff, sizeof(int) * (nc + 1), st));
   if (m == 0) {
     t.n_rng = 0;
     t.rng_start.alloc(1);
     t.rng_len.alloc(1);
-    k_csr_close<<<1, 1, 0, st>>>(nc, t.rng_off.get(), 0);
+    FMM_HIP(hipMemsetAsync(t.rng_off.get(), 0, sizeof(int) * (nc + 1), st));
     FMM_HIP(hipStreamSynchronize(st));
     return;
   }
@@ -534,10 +552,14 @@ void build_p2p_ranges(Tree& t, hipStream_t st) {
   t.rng_start.alloc(t.n_rng);
   t.rng_len.alloc(t.n_rng);
   FMM_HIP(hipMemsetAsync(t.rng_len.get(), 0, sizeof(int) * t.n_rng, st));
+  DevBuf<int> first, end;
+  first.alloc(nc);
+  end.alloc(nc);
+  FMM_HIP(hipMemsetAsync(first.get(), 0, sizeof(int) * nc, st));
+  FMM_HIP(hipMemsetAsync(end.get(), 0, sizeof(int) * nc, st));
   k_rng_fill<<<blocks(m, 256), 256, 0, st>>>(m, k1.get(), b1.get(), t.pcount.get(), fl.get(), ps.get(),
-                                             t.rng_start.get(), t.rng_len.get(), t.rng_off.get());
-  k_csr_close<<<1, 1, 0, st>>>(nc, t.rng_off.get(), t.n_rng);
-  FMM_HIP(hipStreamSynchronize(st));
+                                             t.rng_start.get(), t.rng_len.get(), first.get(), end.get());
+  close_csr(t, first.get(), end.get(), t.rng_off.get(), nc, t.n_rng, st);
 }
 
 // Builds the tree of `t` from caller positions (device, [n*3]).

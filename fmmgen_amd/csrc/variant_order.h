@@ -520,7 +520,7 @@ void bh_order(Tree& t, double* F, hipStream_t st) {
   const int n = (int)t.n;
   FMM_CHECK(7 * t.depth + 1 <= kBHStack, kUnsupported, "tree too deep for the Barnes-Hut walk");
   if (n == 0) return;
-  PhaseTimer pt(t, kTP2P, st);
+  PhaseTimer pt(t, kTBH, st);
   k_bh<P><<<nblk(n, 64), 64, 0, st>>>(n, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(), t.cz.get(),
                                        t.cr.get(), t.pstart.get(), t.pcount.get(), t.px.get(), t.py.get(),
                                        t.pz.get(), t.pS.get(), t.perm.get(), t.M.get(), t.theta, F);

@@ -111,7 +111,7 @@ int fmm_evaluate_phases(fmm_tree_t tree, int mode, int phase_mask, double* d_F, 
  * the totals since the last enable in h_ms[FMM_PHASE_TIMERS] /
  * h_count[FMM_PHASE_TIMERS], indexed 0 prep (gather, zeroing), 1 P2M,
  * 2 M2M, 3 M2L, 4 P2P, 5 L2L, 6 L2P + final sum, 7 learned order
- * selection, 8 judge and reruns, 9 unused (synchronises).  No reference counterpart:
+ * selection, 8 judge and reruns, 9 Barnes-Hut walk (synchronises).  No reference counterpart:
  * the example times the whole evaluation with std::chrono
  * (example/main.cpp:154-162). */
 #define FMM_PHASE_TIMERS 10
