@@ -7,7 +7,9 @@
  * Calls are stream ordered.  The tree build, the exports, the statistics
  * queries and fmm_tree_set_model synchronise; a learned evaluation
  * synchronises once, to launch its far field at the largest selected order
- * (and once more per judge check: the judge's verdict decides a rerun).
+ * (and once more per judge check: the judge's verdict decides a rerun); the
+ * first evaluation of a tree at each order synchronises once, uploading
+ * that order's M2L index table.
  *
  * Reference counterparts (paths in the rpep/fmmgen tree):
  *   fmm_tree_build        build_tree(pos, S, n, ncrit, order, theta)
