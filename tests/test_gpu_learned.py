@@ -329,7 +329,7 @@ def test_learned_other_opening_angles(gpu, theta):
     e, et = _errors(F.cpu().numpy(), Fd)
     _log(f"  theta {theta}: rel field error {e:.3e}, trimmed {et:.3e}, stats {st}")
     assert e <= tol
-    assert st["m2l_mac_fraction"] < 0.6
+    assert st["m2l_mac_fraction"] < 0.9  # not every leaf at the full order
 
 
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
