@@ -57,9 +57,22 @@ def make_inputs(cfg, seed=1):
     return np.ascontiguousarray(pos), np.ascontiguousarray(S)
 
 
-def mac_flops_m2l(p, so):
+def mac_flops_m2l(p, so, harmonic=False):
+    """Useful flops of one M2L pair: the full form's multiply-adds, or the
+    harmonic form's (rows and columns with n_z <= 1)."""
+    if harmonic:
+        return 2 * sum((2 * n + 1) * (p - n + 1) ** 2 for n in range(p + 1))
     from fmmgen_amd.codegen.layout import mac_count_m2l
     return 2 * mac_count_m2l(p, so)
+
+
+def harmonic_m2l(form, mode, so):
+    """Whether the M2L of this mode runs in the harmonic form."""
+    if so != 0 or form == "full" or mode in ("bh",):
+        return False
+    if form == "harmonic":
+        return mode not in ("bf16x3", "learned_bf16x3")
+    return mode in ("f32", "learned")
 
 
 def cpu_baseline(cfg):
@@ -145,9 +158,10 @@ def error_detail(F, Fref, tg, so):
             "mean_abs_rel_err_per_component": [float(v) for v in mean_abs_rel.tolist()]}
 
 
-def roofline_of(phases, info, order, so, mode, m2l_macs=None, config="c4"):
+def roofline_of(phases, info, order, so, mode, m2l_macs=None, config="c4", harmonic=False):
     """Roofline entry of the dominant kernel (P2P: VALU, M2L: MFMA); times are
-    hipEvent phase times on the evaluation stream."""
+    hipEvent phase times on the evaluation stream.  M2L flops are the useful
+    multiply-adds of the form that ran (harmonic: n_z <= 1 rows and columns)."""
     if mode == "bh":  # the walk: per particle M2P and P2P, FP64 VALU
         return {"bound": "valu", "kernel": "BH", "achieved": None, "peak": PEAK["fp64_vector_TF"],
                 "unit": "TFLOP/s", "frac": None, "traffic": None, "avg_ms": phases["bh"],
@@ -158,28 +172,30 @@ def roofline_of(phases, info, order, so, mode, m2l_macs=None, config="c4"):
         peak = PEAK["fp64_vector_TF"] if mode == "plain" else PEAK["fp32_vector_TF"]
         bound = "valu"
     else:
-        flops = 2 * m2l_macs if m2l_macs is not None else mac_flops_m2l(order, so) * info["n_m2l"]
+        flops = 2 * m2l_macs if m2l_macs is not None else mac_flops_m2l(order, so, harmonic) * info["n_m2l"]
         peak = {"plain": PEAK["fp64_mfma_TF"], "f32": PEAK["fp32_mfma_TF"], "learned": PEAK["fp32_mfma_TF"],
                 "bf16x3": PEAK["bf16_mfma_TF"], "learned_bf16x3": PEAK["bf16_mfma_TF"]}[mode]
         bound = "mfma"
     achieved = flops / (phases[dom] * 1e-3) / 1e12
     return {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": measured_traffic(config, mode, dom), "avg_ms": phases[dom],
-            "flop_per_launch": flops}
+            "frac": achieved / peak, "traffic": measured_traffic(config, mode, dom, harmonic), "avg_ms": phases[dom],
+            "flop_per_launch": flops, "m2l_form": ("harmonic" if harmonic else "full") if dom == "M2L" else None}
 
 
-def measured_traffic(config, mode, kernel):
+def measured_traffic(config, mode, kernel, harmonic=False):
     """HBM bytes per evaluation of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement of the same workload
-    (profiles/*_traffic.json, scaled by a copy kernel of known size), or
-    None where no such measurement exists."""
+    rocprofv3 FETCH_SIZE / WRITE_SIZE measurement of the same workload and
+    M2L form (profiles/*_traffic.json, scaled by a copy kernel of known
+    size), or None where no such measurement exists."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"{kernel.lower()}_{mode}_{config}_blobs_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
-    return d["hbm_bytes_per_eval"] if d.get("workload") == config and d.get("mode") == mode else None
+    form = "harmonic" if harmonic else "full"
+    same = d.get("workload") == config and d.get("mode") == mode and d.get("m2l_form", "full") == form
+    return d["hbm_bytes_per_eval"] if same else None
 
 
 def main():
@@ -195,6 +211,8 @@ def main():
     ap.add_argument("--theta", type=float, default=None)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--model", default=None)
+    ap.add_argument("--m2l-form", default="auto", choices=["auto", "full", "harmonic"],
+                    help="M2L operand form of the timed mode (auto: harmonic for f32 / learned)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -248,13 +266,26 @@ def main():
         e_plain, _ = field_error(F, Fref, tg, so)
         print(f"# plain FP64 order {pfix}: {ms_plain:.3f} ms/eval (median), {wall_plain:.3f} (wall mean)",
               file=sys.stderr, flush=True)
-        extra["plain_f64"] = {"order": pfix, "ms_per_step": ms_plain, "ms_per_step_wall": wall_plain,
-                              "value": n / (ms_plain * 1e-3), "field_rel_err": e_plain,
-                              "errors": error_detail(F, Fref, tg, so), "phase_ms": ph_plain}
+        extra["plain_f64"] = {"order": pfix, "m2l_form": "full", "ms_per_step": ms_plain,
+                              "ms_per_step_wall": wall_plain, "value": n / (ms_plain * 1e-3),
+                              "field_rel_err": e_plain, "errors": error_detail(F, Fref, tg, so),
+                              "phase_ms": ph_plain}
+        # the same FP64 method with the harmonic-form M2L (reported beside it)
+        tree.set_m2l_form("harmonic")
+        ms_h, wall_h, ph_h = timed(tree, F, "plain", args.steps, args.warmup)
+        e_h, _ = field_error(F, Fref, tg, so)
+        print(f"# plain FP64 order {pfix}, harmonic M2L: {ms_h:.3f} ms/eval (median), field error {e_h:.3e}",
+              file=sys.stderr, flush=True)
+        extra["plain_f64_harmonic"] = {"order": pfix, "m2l_form": "harmonic", "ms_per_step": ms_h,
+                                       "ms_per_step_wall": wall_h, "value": n / (ms_h * 1e-3),
+                                       "field_rel_err": e_h, "phase_ms": ph_h}
+        tree.set_m2l_form("auto")
         tree.set_order(full)
         mode = "learned"
     else:
         mode = args.mode
+    tree.set_m2l_form(args.m2l_form)
+    harm = harmonic_m2l(args.m2l_form, mode, so)
     if learned:
         from fmmgen_amd import learned as LN
         params, margin, _ = LN.load(args.model or LN.MODEL_PATH)
@@ -269,7 +300,8 @@ def main():
     if stats:
         extra["learned"] = stats
         print(f"# learned stats {stats}", file=sys.stderr, flush=True)
-    roofline = roofline_of(avg, info, cfg["order"], so, mode, stats["m2l_macs"] if stats else None, args.config)
+    macs = (stats["m2l_macs_harmonic"] if harm else stats["m2l_macs"]) if stats else None
+    roofline = roofline_of(avg, info, cfg["order"], so, mode, macs, args.config, harm)
 
     base = None
     if not args.no_cpu_baseline:
@@ -291,7 +323,8 @@ def main():
         "ms_per_step": ms_per_step, "ms_per_step_wall": ms_wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": dtype, "data": "synthetic",
         "config": {"workload": workload, "mode": mode, "n": n, "order": full if learned else cfg["order"],
-                   "theta": cfg["theta"], "ncrit": cfg["ncrit"]},
+                   "theta": cfg["theta"], "ncrit": cfg["ncrit"],
+                   "m2l_form": "harmonic" if harm else "full"},
         "field_rel_err": ferr, "potential_rel_err": perr, "meets_1e-6": bool(ferr <= 1e-6),
         "errors": errs, "timing": "median of hipEvent-timed evaluations on the evaluation stream",
         "tree_build_ms": build_ms, "phase_ms": avg, "tree": info,
@@ -299,6 +332,7 @@ def main():
     }
     if "plain_f64" in extra:
         out["speedup_vs_plain_f64"] = extra["plain_f64"]["ms_per_step"] / ms_per_step
+        out["speedup_vs_plain_f64_harmonic"] = extra["plain_f64_harmonic"]["ms_per_step"] / ms_per_step
     out.update(extra)
     print(json.dumps(out), flush=True)
 

@@ -25,6 +25,8 @@ MLP_NOALLOC_MAX_BATCH = 8192
 PHASE_P2M, PHASE_M2M, PHASE_M2L, PHASE_P2P, PHASE_L2L, PHASE_L2P = 1, 2, 4, 8, 16, 32
 PHASE_ALL = 63
 
+# operand forms of the M2L (fmm_tree_set_m2l_form)
+M2L_FORMS = {"auto": 0, "full": 1, "harmonic": 2}
 OPS = {"P2M": 0, "M2M": 1, "M2L": 2, "L2L": 3, "L2P": 4, "M2P": 5, "P2P": 6, "P2M_S": 7}
 
 # Every symbol include/fmm_hip.h declares.
@@ -35,7 +37,7 @@ EXPORTS = (
     "fmm_op_batch", "fmm_export_cells", "fmm_export_particles", "fmm_export_lists", "fmm_export_expansions",
     "fmm_timing_enable", "fmm_timing_read",
     "fmm_mlp_forward", "fmm_mlp_infer_f16", "fmm_mlp_work_floats", "fmm_mlp_grad", "fmm_mlp_grad_noalloc", "fmm_mlp_adam",
-    "fmm_tree_set_order", "fmm_tree_leaf_features", "fmm_tree_leaf_error", "fmm_tree_leaves",
+    "fmm_tree_set_order", "fmm_tree_set_m2l_form", "fmm_tree_leaf_features", "fmm_tree_leaf_error", "fmm_tree_leaves",
     "fmm_tree_set_model", "fmm_tree_learned_stats", "fmm_tree_leaf_orders", "fmm_tree_field_scale",
 )
 PHASE_NAMES = ("prep", "P2M", "M2M", "M2L", "P2P", "L2L", "L2P", "order", "judge", "bh")
@@ -98,6 +100,7 @@ def load(path=None):
     L.fmm_mlp_grad_noalloc.argtypes = [vp, vp, vp, ci, vp, vp, vp, vp]
     L.fmm_mlp_adam.argtypes = [vp, vp, vp, vp, cf, cf, cf, cf, ci, vp]
     L.fmm_tree_set_order.argtypes = [vp, ci]
+    L.fmm_tree_set_m2l_form.argtypes = [vp, ci]
     L.fmm_tree_leaf_features.argtypes = [vp, vp, vp]
     L.fmm_tree_leaf_error.argtypes = [vp, vp, vp, vp, vp]
     L.fmm_tree_leaves.argtypes = [vp, ll_p]

@@ -410,6 +410,15 @@ int fmm_tree_set_order(fmm_tree_t tree, int order) {
   });
 }
 
+int fmm_tree_set_m2l_form(fmm_tree_t tree, int form) {
+  return guarded([&] {
+    FMM_CHECK(tree, kInvalidArgument, "NULL tree");
+    FMM_CHECK(form == kM2LAuto || form == kM2LFull || form == kM2LHarmonic, kInvalidArgument,
+              "M2L form must be FMM_M2L_AUTO, FMM_M2L_FULL or FMM_M2L_HARMONIC");
+    tree->t.m2l_form = form;
+  });
+}
+
 int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream) {
   return guarded([&] {
     FMM_CHECK(tree && d_feat, kInvalidArgument, "NULL argument");
@@ -492,7 +501,7 @@ int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats) {
     if (!t.lstats_ready && t.lw.get() && t.cell_q.get()) {  // order statistics of the last learned evaluation
       FMM_HIP(hipDeviceSynchronize());
       t.use_tile_order = true;
-      t.lstats[5] = m2l_mac_fraction(t, t.order, nullptr, &t.lstats[6]);
+      t.lstats[5] = m2l_mac_fraction(t, t.order, nullptr, &t.lstats[6], &t.lstats[10]);
       t.use_tile_order = false;
       double lw[kLwCount];
       FMM_HIP(hipMemcpy(lw, t.lw.get(), sizeof(lw), hipMemcpyDeviceToHost));
@@ -501,7 +510,7 @@ int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats) {
       t.lstats[9] = m2l_cell_macs(t, nullptr);
       t.lstats_ready = true;
     }
-    for (int k = 0; k < 10; ++k) h_stats[k] = t.lstats[k];
+    for (int k = 0; k < FMM_LEARNED_STATS_LEN; ++k) h_stats[k] = t.lstats[k];
   });
 }
 

@@ -101,6 +101,12 @@ enum PhaseTimerId : int {
 // offset); every group shares one translation matrix.
 constexpr int kTile = 16;        // target cells per M2L tile (one MFMA column block)
 
+// Operand form of the M2L GEMM: the full graded expansions (the generated
+// operator's sums), or the harmonic layout (folded multipoles, local rows
+// with n_z <= 1, the same sums by the harmonicity of 1/R); automatic = the
+// harmonic layout for the FP32 GEMM, the full form for FP64 and BF16x3.
+enum : int { kM2LAuto = 0, kM2LFull = 1, kM2LHarmonic = 2 };
+
 struct M2LSchedule {
   static constexpr int tile_size = kTile;
   int ntiles = 0;
@@ -119,9 +125,9 @@ struct M2LSchedule {
   // (step, lane) -> D slot tables of the GEMM plans, uploaded once per
   // order and block size (4 or 8 waves)
   static constexpr int kMaxOrder = 16;
-  DevBuf<unsigned> idxtab[kMaxOrder][2];
+  DevBuf<unsigned> idxtab[2][kMaxOrder][2];  // [layout][order][4 or 8 waves]
   DevBuf<unsigned> idxtab32[kMaxOrder];
-  bool idx_ready[kMaxOrder][2] = {}, idx32_ready[kMaxOrder] = {false};
+  bool idx_ready[2][kMaxOrder][2] = {}, idx32_ready[kMaxOrder] = {false};
   // learned order: chunks sorted by their tile's order, and the start of
   // every order's range (kMaxOrder + 1 entries)
   DevBuf<int> class_chunks;
@@ -214,7 +220,7 @@ struct Tree {
   DevBuf<int> judge_slot;
   DevBuf<double> judge_Fd, judge_acc;  // [judge_n * OUT], [4]
   DevBuf<int> judge_fail;              // [1 + ncells] failing count, per cell flag
-  double lstats[10] = {0};
+  double lstats[11] = {0};
   bool lstats_ready = false;       // lstats[5..9] computed for the last learned evaluation
   int class_start_h[M2LSchedule::kMaxOrder + 1] = {0};  // host copy: first work item of every supertile order
 
@@ -225,6 +231,11 @@ struct Tree {
   int ml_order = -1;
   // mixed precision: normalised FP32 expansions, FP64 copies of the top levels
   DevBuf<float> Mf, Lf;
+  // M2L operand form (kM2LAuto / kM2LFull / kM2LHarmonic) and the folded
+  // multipoles and harmonic local rows of a harmonic-layout M2L
+  int m2l_form = 0;
+  DevBuf<float> Mhf, Lhf;
+  DevBuf<double> Mh64, Lh64;
   DevBuf<double> Mtop, Ltop;
   DevBuf<double> Fp2p;            // per-target near-field result, sorted order
   Scratch scratch;

@@ -94,6 +94,6 @@ void raise_all_leaves(Tree& t, int qfull, hipStream_t st);
 // Sum over M2L pairs of C(q + 6, 6) at their supertile's order (the useful
 // multiply-adds of the truncated M2L, into *macs_out), divided by the same
 // sum at order pref; host result.
-double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out = nullptr);
+double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out = nullptr, double* hmacs_out = nullptr);
 
 }  // namespace fmm

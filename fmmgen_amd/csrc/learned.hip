@@ -181,6 +181,13 @@ __device__ __forceinline__ double macs(int q) {  // C(q + 6, 6)
   for (int k = 1; k <= 6; ++k) a = a * (q + k) / k;
   return a;
 }
+// The same pair in the harmonic form: rows and columns with n_z <= 1,
+// sum over n of (2n + 1) (q - n + 1)^2.
+__device__ __forceinline__ double macs_harmonic(int q) {
+  double a = 0.0;
+  for (int n = 0; n <= q; ++n) a += (double)(2 * n + 1) * (q - n + 1) * (q - n + 1);
+  return a;
+}
 
 // M2L pairs of every tile (valid columns of its groups), one block per tile
 __global__ void k_tile_pairs(int ntiles, const int* __restrict__ goff, const int* __restrict__ gsrc,
@@ -203,27 +210,31 @@ __global__ void k_tile_pairs(int ntiles, const int* __restrict__ goff, const int
 // pairs of each tile times C(q + 6, 6)
 __global__ void k_mac_frac(int ntiles, const long long* __restrict__ pairs, const signed char* __restrict__ to,
                            int pref, double* __restrict__ out) {
-  __shared__ double r0[256], r1[256];
+  __shared__ double r0[256], r1[256], r2[256];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double a = 0.0, b = 0.0;
+  double a = 0.0, b = 0.0, c = 0.0;
   if (t < ntiles) {
     const int q = to ? (int)to[t] : pref;
     a = (double)pairs[t] * macs(q);
     b = (double)pairs[t] * macs(pref);
+    c = (double)pairs[t] * macs_harmonic(q);
   }
   r0[threadIdx.x] = a;
   r1[threadIdx.x] = b;
+  r2[threadIdx.x] = c;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       r0[threadIdx.x] += r0[threadIdx.x + s];
       r1[threadIdx.x] += r1[threadIdx.x + s];
+      r2[threadIdx.x] += r2[threadIdx.x + s];
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     atomicAdd(&out[0], r0[0]);
     atomicAdd(&out[1], r1[0]);
+    atomicAdd(&out[2], r2[0]);
   }
 }
 
@@ -722,13 +733,14 @@ void raise_all_leaves(Tree& t, int qfull, hipStream_t st) {
   if (nl > 0) k_fill_i<<<nb(nl, 256), 256, 0, st>>>(nl, t.leaf_q.get(), qfull);
 }
 
-double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out) {
+double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out, double* hmacs_out) {
   const int nt = t.sched.ntiles;
   if (macs_out) *macs_out = 0.0;
+  if (hmacs_out) *hmacs_out = 0.0;
   if (nt == 0) return 1.0;
   DevBuf<double> acc;
-  acc.alloc(2);
-  k_zero_d<<<1, 64, 0, st>>>(2, acc.get());
+  acc.alloc(3);
+  k_zero_d<<<1, 64, 0, st>>>(3, acc.get());
   if (!t.sched.tile_pairs_ready) {  // order independent: once per tile set
     t.sched.tile_pairs.alloc(nt);
     k_tile_pairs<<<nt, 256, 0, st>>>(nt, t.sched.tile_group_off.get(), t.sched.group_src.get(),
@@ -736,10 +748,11 @@ double m2l_mac_fraction(Tree& t, int pref, hipStream_t st, double* macs_out) {
     t.sched.tile_pairs_ready = true;
   }
   k_mac_frac<<<nb(nt, 256), 256, 0, st>>>(nt, t.sched.tile_pairs.get(), t.tile_order_ptr(), pref, acc.get());
-  double h[2] = {0, 0};
+  double h[3] = {0, 0, 0};
   FMM_HIP(hipMemcpyAsync(h, acc.get(), sizeof(h), hipMemcpyDeviceToHost, st));
   FMM_HIP(hipStreamSynchronize(st));
   if (macs_out) *macs_out = h[0];
+  if (hmacs_out) *hmacs_out = h[2];
   return h[1] > 0 ? h[0] / h[1] : 1.0;
 }
 

@@ -52,15 +52,44 @@ __host__ __device__ constexpr int slot_degree(int j) {
   return d;
 }
 
+// Slot layouts of the GEMM's rows (L) and columns (M).
+//   kGraded: every multi-index of degree <= p in the expansions' own graded
+//     order (below).
+//   kHarmonic (charges only): the multi-indices with n_z <= 1, (p + 1)^2
+//     slots; degree d holds slots d^2 .. (d + 1)^2 - 1 in the graded order
+//     restricted to n_z <= 1 (by a = n_y + n_z, then n_z), which keeps the
+//     D slots one MFMA step reads within the 8-bit offsets of its table.
+// Every D_k = d^k (1/R) is harmonic, D_{k+2z} = -D_{k+2x} - D_{k+2y}, so a
+// multipole term M_m with m_z >= 2 acts on the local expansion exactly as
+// -M_m placed at m - 2z + 2x and at m - 2z + 2y (the same degree: every
+// truncation |n| + |m| <= p is kept).  Repeated, M_(a,b,c+2k) adds
+// (-1)^k C(k,i) M to slot (a + 2i, b + 2(k - i), c), c = 0 or 1 (the fold);
+// the translation of the folded multipole gives every L_n exactly, and the
+// L_n with n_z >= 2 follow from the same identity,
+//   L_(a,b,c+2k) = (-1)^k sum_i C(k,i) L_(a+2i, b+2(k-i), c)   (the unfold).
+// The GEMM then has (p + 1)^2 rows and columns instead of C(p + 3, 3).
+enum : int { kGraded = 0, kHarmonic = 1 };
+
+__host__ __device__ constexpr int harm_degree(int j) {
+  int d = 0;
+  while ((d + 1) * (d + 1) <= j) ++d;
+  return d;
+}
+__host__ __device__ constexpr int harm_size(int p) { return (p + 1) * (p + 1); }
+
 // MFMA steps of one translation: row blocks of 16 L slots by slices of KW
 // M slots, the steps whose lowest degrees sum to more than P dropped.
-template <int P, int SO, int KW>
+template <int P, int SO, int KW, int LAY = kGraded>
 struct StepPlan {
-  static constexpr int LS = lsize(P, SO), MS = msize(P, SO);
+  static_assert(LAY == kGraded || SO == 0, "the harmonic layout is for charges");
+  static constexpr int LS = LAY == kHarmonic ? harm_size(P) : lsize(P, SO);
+  static constexpr int MS = LAY == kHarmonic ? harm_size(P) : msize(P, SO);
   static constexpr int NI = (LS + 15) / 16, NK = (MS + KW - 1) / KW;
   static constexpr int OFF = nterms(SO - 1);  // M slot -> full index
+  static constexpr int row_degree(int n) { return LAY == kHarmonic ? harm_degree(n) : slot_degree(n); }
+  static constexpr int col_degree(int m) { return LAY == kHarmonic ? harm_degree(m) : slot_degree(m + OFF); }
   static constexpr bool valid(int I, int K) {
-    return 16 * I < LS && KW * K < MS && slot_degree(16 * I) + slot_degree(KW * K + OFF) <= P;
+    return 16 * I < LS && KW * K < MS && row_degree(16 * I) + col_degree(KW * K) <= P;
   }
   static constexpr int count_steps() {
     int s = 0;
@@ -97,8 +126,8 @@ struct StepPlan {
   static constexpr Tab tab = make();
   static constexpr int NSTEP = tab.base[NK];
 };
-template <int P, int SO>
-using M2LPlan = StepPlan<P, SO, 4>;    // FP32 / FP64 16x16x4 MFMAs
+template <int P, int SO, int LAY = kGraded>
+using M2LPlan = StepPlan<P, SO, 4, LAY>;  // FP32 / FP64 16x16x4 MFMAs
 template <int P, int SO>
 using M2LPlan32 = StepPlan<P, SO, 32>;  // BF16 16x16x32 MFMAs
 
@@ -119,26 +148,54 @@ __host__ __device__ constexpr int multi_slot(int x, int y, int z) {
   const int d = x + y + z, a = d - x;
   return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
 }
+// The same for the harmonic layout (n_z <= 1): within degree d, slot
+// d^2 + r holds a = n_y + n_z = (r + 1) / 2 and n_z = (r + 1) % 2 (r > 0).
+__host__ __device__ constexpr MultiIdx harm_multi(int j) {
+  const int d = harm_degree(j), r = j - d * d;
+  const int a = (r + 1) / 2, z = r == 0 ? 0 : (r + 1) % 2;
+  return MultiIdx{d - a, a - z, z};
+}
+__host__ __device__ constexpr int harm_slot(int x, int y, int z) {
+  const int d = x + y + z, a = y + z;
+  return d * d + (a == 0 ? 0 : 2 * a - 1 + z);
+}
+constexpr bool harm_layout_ok(int p) {
+  for (int j = 0; j < harm_size(p); ++j) {
+    const MultiIdx m = harm_multi(j);
+    if (m.x < 0 || m.y < 0 || m.z < 0 || m.z > 1 || m.x + m.y + m.z != harm_degree(j) ||
+        harm_slot(m.x, m.y, m.z) != j)
+      return false;
+  }
+  return true;
+}
+static_assert(harm_layout_ok(15), "harmonic slots: a bijection onto the multi-indices with n_z <= 1");
+
+template <int LAY>
+__host__ __device__ constexpr MultiIdx lay_multi(int j, int off) {
+  return LAY == kHarmonic ? harm_multi(j) : slot_multi(j + off);
+}
 
 // D-vector slot (full index - OFF) lane l reads at step (I, K): the entry
 // T[16 I + l%16][4 K + l/16] = D[n + m], or -1 outside the expansion.
-template <int P, int SO>
+// (D is always in the graded layout.)
+template <int P, int SO, int LAY = kGraded>
 __host__ __device__ constexpr int step_lane_slot(int I, int K, int l) {
-  constexpr int LS = lsize(P, SO), MS = msize(P, SO), OFF = nterms(SO - 1);
+  using Plan = M2LPlan<P, SO, LAY>;
+  constexpr int LS = Plan::LS, MS = Plan::MS, OFF = Plan::OFF;
   const int n = 16 * I + (l & 15), m = 4 * K + (l >> 4);
   if (n >= LS || m >= MS) return -1;
-  const MultiIdx a = slot_multi(n), b = slot_multi(m + OFF);
+  const MultiIdx a = lay_multi<LAY>(n, 0), b = lay_multi<LAY>(m, OFF);
   if (a.x + a.y + a.z + b.x + b.y + b.z > P) return -1;
   return multi_slot(a.x + b.x, a.y + b.y, a.z + b.z) - OFF;
 }
 
 // Lowest D slot a step reads: the index table holds 8-bit offsets from it
 // (255 marks an empty entry), the base is a compile-time immediate.
-template <int P, int SO>
+template <int P, int SO, int LAY = kGraded>
 __host__ __device__ constexpr int step_base(int I, int K) {
   int b = 1 << 30;
   for (int l = 0; l < 64; ++l) {
-    const int v = step_lane_slot<P, SO>(I, K, l);
+    const int v = step_lane_slot<P, SO, LAY>(I, K, l);
     if (v >= 0 && v < b) b = v;
   }
   return b == (1 << 30) ? 0 : b;
@@ -281,9 +338,9 @@ struct WavePlanRR {
 
 // The plan a kernel of order P uses (measured: the balanced plan is faster
 // from order 7 up, equal at 6).
-template <int P, int SO, int W, int WV>
-using WavePlanFor =
-    std::conditional_t<(P >= 6), WavePlan<M2LPlan<P, SO>, W, WV>, WavePlanRR<M2LPlan<P, SO>, W, WV>>;
+template <int P, int SO, int W, int WV, int LAY = kGraded>
+using WavePlanFor = std::conditional_t<(P >= 6), WavePlan<M2LPlan<P, SO, LAY>, W, WV>,
+                                       WavePlanRR<M2LPlan<P, SO, LAY>, W, WV>>;
 template <int P, int SO, int W, int WV>
 using WavePlan32For = WavePlan<M2LPlan32<P, SO>, W, WV>;
 
@@ -302,20 +359,20 @@ constexpr int wave_steps32() {
 // to a 32-bit word (one LDS read serves four steps): word d of lane l of
 // wave wv sits at wave_words<..., wv>() + 64 d + l and holds steps
 // 4d .. 4d + 3 of the wave's order in bytes 0 .. 3.
-template <int P, int SO, int W, int WV>
+template <int P, int SO, int W, int WV, int LAY = kGraded>
 constexpr int wave_words() {
   if constexpr (WV == 0) {
     return 0;
   } else {
-    return wave_words<P, SO, W, WV - 1>() + 64 * ((WavePlanFor<P, SO, W, WV - 1>::tab.n + 3) / 4);
+    return wave_words<P, SO, W, WV - 1, LAY>() + 64 * ((WavePlanFor<P, SO, W, WV - 1, LAY>::tab.n + 3) / 4);
   }
 }
 
 // Waves per block: the LDS tables of a chunk are shared by W waves.
-template <int P, int SO, typename T>
+template <int P, int SO, typename T, int LAY = kGraded>
 constexpr int m2l_waves() {
-  using Plan = M2LPlan<P, SO>;
-  constexpr int lds = Plan::NSTEP * 64 + (int)sizeof(T) * (kTile * (Plan::MS | 1) + Plan::MS + 256);
+  using Plan = M2LPlan<P, SO, LAY>;
+  constexpr int lds = Plan::NSTEP * 64 + (int)sizeof(T) * (kTile * (Plan::MS | 1) + msize(P, SO) + 256);
   return lds > 80 * 1024 ? 8 : 4;
 }
 
@@ -328,10 +385,12 @@ __device__ __forceinline__ void dispatch_wave(int w, F&& f) {
 
 // tile_order (optional, per tile): the order q <= P the pairs of the tile's
 // groups are truncated at (D entries of degree > q are zeroed).  chunk_list
-// (optional): the chunks to run (else 0 .. nchunks-1).  mstride / lstride:
-// row strides of M and the D table / of L (the sizes of a higher order
-// when the expansions are stored for one: the layout is a prefix).
-template <int P, int SO, typename T, bool SCALED, int W>
+// (optional): the chunks to run (else 0 .. nchunks-1).  dstride / mstride /
+// lstride: row strides of the D table / of M / of L (the sizes of a higher
+// order when the expansions are stored for one: every layout is a prefix).
+// LAY: the layout of M's and L's rows (the folded multipoles and the
+// unfolded local expansions for kHarmonic); D is graded either way.
+template <int P, int SO, typename T, bool SCALED, int W, int LAY = kGraded>
 __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, const int* __restrict__ tile_cells,
                                                      const int* __restrict__ chunk_tile,
                                                      const int* __restrict__ chunk_g0,
@@ -341,29 +400,33 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
                                                      const int* __restrict__ key_dlevel, const T* __restrict__ Dtab,
                                                      const unsigned* __restrict__ idxtab,
                                                      const signed char* __restrict__ tile_order,
-                                                     const int* __restrict__ chunk_list, int mstride, int lstride,
-                                                     const T* __restrict__ M, T* __restrict__ L) {
-  using Plan = M2LPlan<P, SO>;
-  constexpr int MS = Plan::MS, LS = Plan::LS, NI = Plan::NI, NST = Plan::NSTEP;
+                                                     const int* __restrict__ chunk_list, int dstride, int mstride,
+                                                     int lstride, const T* __restrict__ M, T* __restrict__ L) {
+  using Plan = M2LPlan<P, SO, LAY>;
+  constexpr int MS = Plan::MS, LS = Plan::LS;
+  constexpr int MD = msize(P, SO);       // D vector slots
   constexpr int MSP = (MS | 1);          // odd row stride of the staged multipoles: conflict-free B reads
   constexpr int NJ = (MS + 63) / 64;     // 64-slot segments of one column
   constexpr int CPW = kTile / W;         // columns staged by each wave
-  constexpr int DJ = (NJ + W - 1) / W;   // D segments staged by each wave
+  constexpr int DJ = ((MD + 63) / 64 + W - 1) / W;  // D segments staged by each wave
   using V4 = typename Vec4<T>::type;
-  constexpr int NWORDS = wave_words<P, SO, W, W>();
+  constexpr int NWORDS = wave_words<P, SO, W, W, LAY>();
   __shared__ unsigned sidx[NWORDS];  // offsets from each step's base slot, four steps a word
-  __shared__ T sD[MS + 256];                 // slots MS .. MS + 255 stay zero
+  __shared__ T sD[MD + 256];                 // slots MD .. MD + 255 stay zero
   __shared__ T sB[kTile * MSP + 4];          // tail slots stay zero
-  __shared__ signed char sdeg[MS];
+  __shared__ signed char sdeg[MD];           // degree of each D slot
+  __shared__ signed char sdegB[LAY == kHarmonic && SCALED ? MS : 1];  // degree of each M slot
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int ci = xcd_tile(blockIdx.x, nblocks);
   if (ci >= nchunks) return;  // uniform over the block
   const int chunk = chunk_list ? chunk_list[ci] : ci;
   const int tile = chunk_tile[chunk];
   for (int i = tid; i < NWORDS; i += 64 * W) sidx[i] = idxtab[i];
-  for (int i = tid; i < 256; i += 64 * W) sD[MS + i] = T(0);
+  for (int i = tid; i < 256; i += 64 * W) sD[MD + i] = T(0);
   for (int i = tid; i < kTile * MSP + 4; i += 64 * W) sB[i] = T(0);
-  for (int j = tid; j < MS; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
+  for (int j = tid; j < MD; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
+  if constexpr (LAY == kHarmonic && SCALED)
+    for (int j = tid; j < MS; j += 64 * W) sdegB[j] = (signed char)harm_degree(j);
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
   const int q = tile_order ? (int)tile_order[tile] : P;
@@ -371,7 +434,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 
   dispatch_wave<W>(w, [&](auto WV) {
     constexpr int wv = WV;
-    using WP = WavePlanFor<P, SO, W, wv>;
+    using WP = WavePlanFor<P, SO, W, wv, LAY>;
     constexpr int NIW = WP::tab.nrow;  // accumulators: one per row block touched
     constexpr int NSW = WP::tab.n;
     V4 acc[NIW > 0 ? NIW : 1];
@@ -379,15 +442,15 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
     for (int i = 0; i < (NIW > 0 ? NIW : 1); ++i) acc[i] = V4{0, 0, 0, 0};
     T rD[DJ], rB[CPW][NJ];
     // loads are issued unconditionally (empty columns read cell 0, slots
-    // past MS read the row's last slot) and masked afterwards: no branch
+    // past the row read its last slot) and masked afterwards: no branch
     // around a load
     auto prefetch = [&](int g) {
       const int key = group_key[g];
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        const T v = Dtab[(size_t)key * mstride + (j < MS ? j : MS - 1)];
-        rD[i] = j < MS ? v : T(0);
+        const T v = Dtab[(size_t)key * dstride + (j < MD ? j : MD - 1)];
+        rD[i] = j < MD ? v : T(0);
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c) {
@@ -408,7 +471,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        if (j < MS) sD[j] = sdeg[j] <= qc ? rD[i] : T(0);
+        if (j < MD) sD[j] = sdeg[j] <= qc ? rD[i] : T(0);
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c)
@@ -417,7 +480,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
           const int j = 64 * i + lane;
           if (j < MS) {
             T v = rB[c][i];
-            if (SCALED && dl != 0) v = ldexpf(v, dl * sdeg[j]);
+            if (SCALED && dl != 0) v = ldexpf(v, dl * (LAY == kHarmonic ? sdegB[j] : sdeg[j]));
             sB[(wv + W * c) * MSP + j] = v;
           }
         }
@@ -428,7 +491,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
       // degree zeroed; the learned evaluation launches each tile order with
       // the kernel of that order, so no step is wasted there)
       constexpr int kBatch = 16;  // a multiple of 4: a batch starts a packed word
-      constexpr int WW = wave_words<P, SO, W, wv>();
+      constexpr int WW = wave_words<P, SO, W, wv, LAY>();
       unsigned word = 0;
       static_for<(NSW + kBatch - 1) / kBatch>([&](auto bb) {
         {
@@ -438,13 +501,13 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
             if constexpr (i < NSW) {
               constexpr int s = WP::tab.s[i];
               constexpr int K = Plan::tab.sK[s], I = Plan::tab.sI[s];
-              constexpr int base = step_base<P, SO>(I, K);
+              constexpr int base = step_base<P, SO, LAY>(I, K);
               if constexpr (i % 4 == 0) word = sidx[WW + 64 * (i / 4) + lane];
               const int off = (int)((word >> (8 * (i % 4))) & 255u);
               av[u] = sD[base + off];
               // an empty entry (offset 255) reads a zero slot when the
-              // step's base is within 255 of MS, else it is masked here
-              if constexpr (MS - base > 255) av[u] = off == 255 ? T(0) : av[u];
+              // step's base is within 255 of MD, else it is masked here
+              if constexpr (MD - base > 255) av[u] = off == 255 ? T(0) : av[u];
               constexpr int KP = (u > 0) ? Plan::tab.sK[WP::tab.s[i > 0 ? i - 1 : 0]] : -1;
               if constexpr (K == KP) bv[u] = bv[u > 0 ? u - 1 : 0];
               else bv[u] = sB[col * MSP + 4 * K + kk];
@@ -474,6 +537,62 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
       }
     });
   });
+}
+
+// Pascal's triangle up to row 15 (the fold and unfold coefficients).
+struct Binom {
+  int c[16][16];
+  constexpr Binom() : c{} {
+    for (int n = 0; n < 16; ++n) {
+      c[n][0] = 1;
+      for (int k = 1; k <= n; ++k) c[n][k] = c[n - 1][k - 1] + (k < n ? c[n - 1][k] : 0);
+    }
+  }
+};
+
+// Fold (see the harmonic layout above): Mh[cell][j] for the (P + 1)^2
+// harmonic slots j from the graded multipole M[cell] (stride ms), one
+// thread per (cell, slot), summed in FP64.
+template <int P, typename T>
+__global__ __launch_bounds__(256) void k_harm_fold(int nc, const T* __restrict__ M, int ms, T* __restrict__ Mh,
+                                                   int hs) {
+  constexpr int HS = harm_size(P);
+  constexpr Binom B{};
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nc * HS) return;
+  const int cell = (int)(i / HS), j = (int)(i % HS);
+  const MultiIdx h = harm_multi(j);
+  const T* m = M + (size_t)cell * ms;
+  double v = 0.0;
+  for (int k = 0; 2 * k <= h.x + h.y; ++k) {
+    double sk = 0.0;
+    for (int a = 0; a <= k; ++a) {  // source (x - 2a, y - 2(k - a), z + 2k)
+      const int sx = h.x - 2 * a, sy = h.y - 2 * (k - a);
+      if (sx >= 0 && sy >= 0) sk += (double)B.c[k][a] * (double)m[multi_slot(sx, sy, h.z + 2 * k)];
+    }
+    v += (k & 1) ? -sk : sk;
+  }
+  Mh[(size_t)cell * hs + j] = (T)v;
+}
+
+// Unfold: L[cell][n] += the graded local expansion of the harmonic rows
+// Lh[cell] (stride hs), one thread per (cell, graded slot), in FP64.
+template <int P, typename T>
+__global__ __launch_bounds__(256) void k_harm_unfold(int nc, const T* __restrict__ Lh, int hs, T* __restrict__ L,
+                                                     int ls) {
+  constexpr int LS = nterms(P);
+  constexpr Binom B{};
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)nc * LS) return;
+  const int cell = (int)(i / LS), n = (int)(i % LS);
+  const MultiIdx t = slot_multi(n);
+  const T* h = Lh + (size_t)cell * hs;
+  const int k = t.z >> 1, c = t.z & 1;
+  double v = 0.0;
+  for (int a = 0; a <= k; ++a) v += (double)B.c[k][a] * (double)h[harm_slot(t.x + 2 * a, t.y + 2 * (k - a), c)];
+  if (k & 1) v = -v;
+  T* o = L + (size_t)cell * ls + n;
+  *o = (T)((double)*o + v);
 }
 
 // ---------------------------------------------------------------- BF16x3
@@ -708,32 +827,32 @@ std::vector<unsigned> m2l_index_table32() {
 // Host side: the index table of a W-wave block (layout at wave_words):
 // 8-bit offsets of each lane's D slot from its step's base slot, 255 for
 // an empty entry.
-template <int P, int SO, int W, int WV>
+template <int P, int SO, int W, int WV, int LAY>
 void fill_wave_index(std::vector<unsigned>& t) {
   if constexpr (WV < W) {
-    using WP = WavePlanFor<P, SO, W, WV>;
-    using Plan = M2LPlan<P, SO>;
-    const size_t w0 = (size_t)wave_words<P, SO, W, WV>();
+    using WP = WavePlanFor<P, SO, W, WV, LAY>;
+    using Plan = M2LPlan<P, SO, LAY>;
+    const size_t w0 = (size_t)wave_words<P, SO, W, WV, LAY>();
     for (int i = 0; i < WP::tab.n; ++i) {
       const int st = WP::tab.s[i];
       const int I = Plan::tab.sI[st], K = Plan::tab.sK[st];
-      const int base = step_base<P, SO>(I, K);
+      const int base = step_base<P, SO, LAY>(I, K);
       for (int l = 0; l < 64; ++l) {
-        const int v = step_lane_slot<P, SO>(I, K, l);
+        const int v = step_lane_slot<P, SO, LAY>(I, K, l);
         if (v >= 0 && v - base > 254) throw Error{kUnsupported, "M2L step spans more than 255 D slots"};
         const unsigned b = v < 0 ? 255u : (unsigned)(v - base);
         unsigned& word = t[w0 + 64 * (size_t)(i / 4) + l];
         word = (word & ~(255u << (8 * (i % 4)))) | (b << (8 * (i % 4)));
       }
     }
-    fill_wave_index<P, SO, W, WV + 1>(t);
+    fill_wave_index<P, SO, W, WV + 1, LAY>(t);
   }
 }
 
-template <int P, int SO, int W>
+template <int P, int SO, int W, int LAY = kGraded>
 std::vector<unsigned> m2l_index_table() {
-  std::vector<unsigned> t((size_t)wave_words<P, SO, W, W>(), 0xffffffffu);
-  fill_wave_index<P, SO, W, 0>(t);
+  std::vector<unsigned> t((size_t)wave_words<P, SO, W, W, LAY>(), 0xffffffffu);
+  fill_wave_index<P, SO, W, 0, LAY>(t);
   return t;
 }
 

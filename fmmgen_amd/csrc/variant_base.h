@@ -437,9 +437,9 @@ void near_field(Tree& t, int mode, hipStream_t st) {
 
 template <int P>
 struct OrderTable {
-  static void m2l(int order, Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hipStream_t st) {
-    if (order == P) m2l_class<P>(t, mode, chunks, nch, ms, ls, st);
-    else OrderTable<P + 1>::m2l(order, t, mode, chunks, nch, ms, ls, st);
+  static void m2l(int order, Tree& t, int mode, const int* chunks, int nch, int ps, hipStream_t st) {
+    if (order == P) m2l_class<P>(t, mode, chunks, nch, ps, st);
+    else OrderTable<P + 1>::m2l(order, t, mode, chunks, nch, ps, st);
   }
   static void far(int order, Tree& t, int mode, int mask, hipStream_t st) {
     if (order == P) far_field_order<P>(t, mode, mask, st);
@@ -461,7 +461,7 @@ struct OrderTable {
 };
 template <>
 struct OrderTable<PMAX> {
-  static void m2l(int, Tree&, int, const int*, int, int, int, hipStream_t) {
+  static void m2l(int, Tree&, int, const int*, int, int, hipStream_t) {
     throw Error{kUnsupported, "order not compiled"};
   }
   static void far(int, Tree&, int, int, hipStream_t) { throw Error{kUnsupported, "order not compiled"}; }
@@ -525,10 +525,9 @@ void gather_sources(Tree& t, hipStream_t st) {
 // field's order P (row strides of order P).
 void m2l_by_class(Tree& t, int mode, int P, hipStream_t st) {
   const int* start = t.class_start_h;
-  const int ms = msize(P, SO), ls = lsize(P, SO);
   for (int q = PMIN; q <= P && q < M2LSchedule::kMaxOrder; ++q) {
     const int nch = start[q + 1] - start[q];
-    if (nch > 0) OrderTable<PMIN>::m2l(q, t, mode, t.sched.class_chunks.get() + start[q], nch, ms, ls, st);
+    if (nch > 0) OrderTable<PMIN>::m2l(q, t, mode, t.sched.class_chunks.get() + start[q], nch, P, st);
   }
 }
 

@@ -33,8 +33,16 @@ void op_batch_order(int op, int n, const double* xyz, const double* in, int in_s
                     int out_stride, hipStream_t st);
 template <int P>
 void bh_order(Tree& t, double* F, hipStream_t st);
+// M2L of the listed chunks at order Q on expansions stored at order ps.
 template <int Q>
-void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hipStream_t st);
+void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ps, hipStream_t st);
+// Whether the M2L of an evaluation in this mode runs in the harmonic layout.
+inline bool m2l_harmonic(const Tree& t, int mode) {
+  if (SO != 0) return false;
+  if (t.m2l_form == kM2LFull) return false;
+  if (t.m2l_form == kM2LHarmonic) return mode != 2;
+  return mode == 1;  // automatic: the FP32 GEMM of the mixed and learned modes
+}
 
 // Order-independent phases (defined in the base translation unit).
 void near_field(Tree& t, int mode, hipStream_t st);

@@ -116,20 +116,38 @@ __global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const doubl
   for (int j = 0; j < MS; ++j) Dtab[(size_t)k * MS + j] = (T)D[j];
 }
 
-template <int P, int W>
+template <int P, int W, int LAY = kGraded>
 const unsigned* ensure_index_table(Tree& t, hipStream_t st) {
   static_assert(W == 4 || W == 8, "blocks of 4 or 8 waves");
   M2LSchedule& S = t.sched;
   const int wi = W == 8 ? 1 : 0;
-  if (!S.idx_ready[P][wi]) {
-    const std::vector<unsigned> tab = m2l_index_table<P, SO, W>();
-    S.idxtab[P][wi].alloc(tab.size());
-    FMM_HIP(hipMemcpyAsync(S.idxtab[P][wi].get(), tab.data(), sizeof(unsigned) * tab.size(),
+  if (!S.idx_ready[LAY][P][wi]) {
+    const std::vector<unsigned> tab = m2l_index_table<P, SO, W, LAY>();
+    S.idxtab[LAY][P][wi].alloc(tab.size());
+    FMM_HIP(hipMemcpyAsync(S.idxtab[LAY][P][wi].get(), tab.data(), sizeof(unsigned) * tab.size(),
                            hipMemcpyHostToDevice, st));
     FMM_HIP(hipStreamSynchronize(st));
-    S.idx_ready[P][wi] = true;
+    S.idx_ready[LAY][P][wi] = true;
   }
-  return S.idxtab[P][wi].get();
+  return S.idxtab[LAY][P][wi].get();
+}
+
+// Harmonic-layout M2L around the GEMM: fold every cell's multipole (M,
+// graded, stride ms) into Mh, clear Lh (stride of order ps), then, after
+// the GEMM, add the unfolded rows into L (graded, stride ls).
+template <int PS, typename T>
+void harm_fold(Tree& t, const T* M, int ms, DevBuf<T>& Mh, DevBuf<T>& Lh, hipStream_t st) {
+  const int nc = t.ncells, hs = harm_size(PS);
+  Mh.alloc((size_t)nc * hs);
+  Lh.alloc((size_t)nc * hs);
+  const long long n = (long long)nc * hs;
+  k_harm_fold<PS, T><<<(int)((n + 255) / 256), 256, 0, st>>>(nc, M, ms, Mh.get(), hs);
+  FMM_HIP(hipMemsetAsync(Lh.get(), 0, sizeof(T) * n, st));
+}
+template <int PS, typename T>
+void harm_unfold(Tree& t, const DevBuf<T>& Lh, T* L, int ls, hipStream_t st) {
+  const long long n = (long long)t.ncells * nterms(PS);
+  k_harm_unfold<PS, T><<<(int)((n + 255) / 256), 256, 0, st>>>(t.ncells, Lh.get(), harm_size(PS), L, ls);
 }
 
 template <int P, int W>
@@ -150,15 +168,28 @@ template <int P>
 void m2l_plain(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (S.ngroups == 0) return;
-  constexpr int W = m2l_waves<P, SO, double>();
-  const unsigned* idx = ensure_index_table<P, W>(t, st);
   S.Dtab64.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, double, false><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab64.get());
   const int grid = ((S.nchunks + 7) / 8) * 8;
+  if constexpr (SO == 0) {
+    if (m2l_harmonic(t, 0)) {
+      constexpr int W = m2l_waves<P, SO, double, kHarmonic>();
+      const unsigned* idx = ensure_index_table<P, W, kHarmonic>(t, st);
+      harm_fold<P, double>(t, t.M.get(), Op<P>::MS, t.Mh64, t.Lh64, st);
+      k_m2l_gemm<P, SO, double, false, W, kHarmonic><<<grid, 64 * W, 0, st>>>(
+          S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(),
+          S.group_key.get(), S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(), idx, t.tile_order_ptr(), nullptr,
+          Op<P>::MS, harm_size(P), harm_size(P), t.Mh64.get(), t.Lh64.get());
+      harm_unfold<P, double>(t, t.Lh64, t.L.get(), Op<P>::LS, st);
+      return;
+    }
+  }
+  constexpr int W = m2l_waves<P, SO, double>();
+  const unsigned* idx = ensure_index_table<P, W>(t, st);
   k_m2l_gemm<P, SO, double, false, W><<<grid, 64 * W, 0, st>>>(
       S.nchunks, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
       S.group_src.get(), S.key_dlevel.get(), S.Dtab64.get(), idx, t.tile_order_ptr(), nullptr, Op<P>::MS,
-      Op<P>::LS, t.M.get(), t.L.get());
+      Op<P>::MS, Op<P>::LS, t.M.get(), t.L.get());
 }
 
 // L2L (example/calculate.cpp:219-234), x = child - parent, one thread per
@@ -236,20 +267,27 @@ void m2l_mixed(Tree& t, int mode, hipStream_t st) {
   if (S.ngroups == 0) return;
   S.Dtab32.alloc((size_t)S.nkeys * Op<P>::MS);
   k_dtab<P, float, true><<<nblk(S.nkeys, 64), 64, 0, st>>>(S.nkeys, S.key_dx.get(), S.key_r.get(), S.Dtab32.get());
+  const bool harm = m2l_harmonic(t, mode);
+  if constexpr (SO == 0)
+    if (harm) harm_fold<P, float>(t, t.Mf.get(), Op<P>::MS, t.Mhf, t.Lhf, st);
   if (t.use_tile_order && t.m2l_classes) {  // learned order: one launch per tile-order class
     t.m2l_classes(t, mode, P, st);
-    return;
+  } else {
+    m2l_class<P>(t, mode, nullptr, S.nchunks, P, st);
   }
-  m2l_class<P>(t, mode, nullptr, S.nchunks, Op<P>::MS, Op<P>::LS, st);
+  if constexpr (SO == 0)
+    if (harm) harm_unfold<P, float>(t, t.Lhf, t.Lf.get(), Op<P>::LS, st);
 }
 
 // FP32 / BF16x3 M2L of the listed chunks (all when chunks == nullptr) at
-// order Q, on expansions and D vectors stored with row strides ms / ls.
+// order Q, on expansions and D vectors stored at order ps (the layouts are
+// prefixes: row strides of order ps).
 template <int Q>
-void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hipStream_t st) {
+void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ps, hipStream_t st) {
   M2LSchedule& S = t.sched;
   if (nch == 0) return;
   const int grid = ((nch + 7) / 8) * 8;
+  const int ms = msize(ps, SO), ls = lsize(ps, SO);
   if (mode == 2) {  // BF16x3 split precision
     constexpr int W32 = m2l_waves32<Q, SO>();
     const unsigned* idx = ensure_index_table32<Q, W32>(t, st);
@@ -259,12 +297,23 @@ void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ms, int ls, hi
                                              ms, ls, t.Mf.get(), t.Lf.get());
     return;
   }
+  if constexpr (SO == 0) {
+    if (m2l_harmonic(t, mode)) {
+      constexpr int W = m2l_waves<Q, SO, float, kHarmonic>();
+      const unsigned* idx = ensure_index_table<Q, W, kHarmonic>(t, st);
+      k_m2l_gemm<Q, SO, float, true, W, kHarmonic><<<grid, 64 * W, 0, st>>>(
+          nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
+          S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks, ms,
+          harm_size(ps), harm_size(ps), t.Mhf.get(), t.Lhf.get());
+      return;
+    }
+  }
   constexpr int W = m2l_waves<Q, SO, float>();
   const unsigned* idx = ensure_index_table<Q, W>(t, st);
   k_m2l_gemm<Q, SO, float, true, W><<<grid, 64 * W, 0, st>>>(
       nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
-      S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks, ms, ls, t.Mf.get(),
-      t.Lf.get());
+      S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks, ms, ms, ls,
+      t.Mf.get(), t.Lf.get());
 }
 
 template <int P>
@@ -576,6 +625,6 @@ void op_batch_order(int op, int n, const double* xyz, const double* in, int is, 
   template void l2p_final_order<P>(Tree&, int, int, double*, hipStream_t);                              \
   template void op_batch_order<P>(int, int, const double*, const double*, int, double*, int, hipStream_t); \
   template void bh_order<P>(Tree&, double*, hipStream_t);                                                 \
-  template void m2l_class<P>(Tree&, int, const int*, int, int, int, hipStream_t);                         \
+  template void m2l_class<P>(Tree&, int, const int*, int, int, hipStream_t);                         \
   }                                                                                                     \
   }

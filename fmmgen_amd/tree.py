@@ -63,6 +63,17 @@ class Tree:
                 pass
             self._h = None
 
+    def set_m2l_form(self, form):
+        """Operand form of the M2L translation: "auto" (harmonic for the FP32
+        GEMM of the f32 and learned modes, full otherwise), "full" (the
+        generated operator's sums over every multi-index) or "harmonic"
+        (folded multipoles and the local rows with n_z <= 1: the same
+        expansions by the harmonicity of 1/R, (p+1)^2 rows and columns
+        instead of C(p+3,3); the BF16x3 modes keep the full form)."""
+        if form not in _lib.M2L_FORMS:
+            raise ValueError(f"M2L form must be one of {sorted(_lib.M2L_FORMS)}")
+        _lib.check(_lib.load().fmm_tree_set_m2l_form(self._h, _lib.M2L_FORMS[form]))
+
     # ----------------------------------------------------------- learned order
     def set_order(self, order):
         """Evaluation order (the tree and lists do not depend on it)."""
@@ -112,10 +123,10 @@ class Tree:
         _lib.check(_lib.load().fmm_tree_set_model(self._h, _ptr(params.contiguous()), float(tol), float(margin)))
 
     def learned_stats(self):
-        a = (ctypes.c_double * 10)()
+        a = (ctypes.c_double * 11)()
         _lib.check(_lib.load().fmm_tree_learned_stats(self._h, a))
         keys = ("order_max", "judge_failures", "reruns", "judge_rel_err", "judge_targets", "m2l_mac_fraction",
-                "m2l_macs", "predicted_rel_err", "field_scale", "m2l_cell_macs")
+                "m2l_macs", "predicted_rel_err", "field_scale", "m2l_cell_macs", "m2l_macs_harmonic")
         return dict(zip(keys, list(a)))
 
     def leaf_orders(self):

@@ -202,6 +202,20 @@ int fmm_mlp_adam(const float* d_grad, float* d_params, float* d_m, float* d_v, f
 /* Evaluation order of the tree (any compiled order: the tree and lists
  * do not depend on it); the learned modes use it as the full order. */
 int fmm_tree_set_order(fmm_tree_t tree, int order);
+/* Operand form of the M2L translation (no counterpart in the reference,
+ * whose M2L is the generated operator, fmmgen/expansions.py:144-172):
+ *   FMM_M2L_AUTO (default): harmonic for the FP32 GEMM (the f32 and learned
+ *     modes), full for plain FP64 and the BF16x3 modes;
+ *   FMM_M2L_FULL: the generated operator's sums over every multi-index;
+ *   FMM_M2L_HARMONIC: the same local expansions from folded multipoles and
+ *     the rows with n_z <= 1 only ((p+1)^2 instead of C(p+3,3) per side,
+ *     exact by the harmonicity of 1/R; results differ from the full form by
+ *     rounding); the BF16x3 modes keep the full form.
+ * Charges only (source order 0): other variants always use the full form. */
+#define FMM_M2L_AUTO 0
+#define FMM_M2L_FULL 1
+#define FMM_M2L_HARMONIC 2
+int fmm_tree_set_m2l_form(fmm_tree_t tree, int form);
 /* Leaf features d_feat [nleaves*FMM_MLP_FEATURES], leaves in canonical
  * order (computes the field scale first; needs one previous evaluation). */
 int fmm_tree_leaf_features(fmm_tree_t tree, float* d_feat, void* stream);
@@ -222,7 +236,7 @@ int fmm_tree_leaves(fmm_tree_t tree, long long* h_leaves);
  * sources as a key inside each tile class (a tile runs at the largest order
  * of its cells); synchronises the default stream. */
 int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, double margin);
-/* Statistics of the last learned evaluation, h_stats [10]: largest order
+/* Statistics of the last learned evaluation, h_stats [FMM_LEARNED_STATS_LEN]: largest order
  * used, judge targets above tol * rms |F| (a rerun happens when the
  * sample's rel. field error exceeds tol), reruns (0: none, 1: the
  * failing leaves (or all leaves) at the full order, 2: a partial rerun
@@ -231,7 +245,9 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
  * relative to the tree's order, M2L useful multiply-adds (sum over pairs of
  * C(q + 6, 6) at the pair's order), predicted error of the chosen orders
  * relative to F_s, the field scale F_s, M2L multiply-adds at the target
- * cells' own orders (without the tile maximum). */
+ * cells' own orders (without the tile maximum), M2L useful multiply-adds
+ * in the harmonic form (sum over pairs of sum_n (2n+1)(q-n+1)^2). */
+#define FMM_LEARNED_STATS_LEN 11
 int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats);
 /* Per-leaf orders of the last learned evaluation: h_order [nleaves]. */
 int fmm_tree_leaf_orders(fmm_tree_t tree, int* h_order);

@@ -252,6 +252,59 @@ def test_fmm_mixed_matches_oracle(gpu, kind, n, ncrit, order, theta, so, mode):
     assert util.rel_norm(Fm.cpu().numpy()[:, fo:], F64.cpu().numpy()[:, fo:]) < 1e-6
 
 
+HARMONIC_CASES = [
+    # kind, n, ncrit, orders, theta
+    ("blobs", 20000, 64, (2, 3, 4, 5, 6, 7), 0.5),
+    ("uniform", 20000, 64, (8, 9, 10, 11, 12), 0.5),
+    ("mixed", 30000, 128, (10,), 0.5),
+]
+
+
+@pytest.mark.parametrize("kind,n,ncrit,orders,theta", HARMONIC_CASES)
+def test_fmm_harmonic_m2l_f64_matches_oracle(gpu, kind, n, ncrit, orders, theta):
+    """The harmonic-layout M2L (folded multipoles, local rows with n_z <= 1,
+    unfolded afterwards) in FP64 gives the oracle's expansions to rounding
+    at every compiled order: the same sums as the generated operator."""
+    from fmmgen_amd import build_tree
+    pos = _positions(kind, n, 21)
+    S = np.random.default_rng(22).uniform(-1, 1, (n, 1))
+    gt = build_tree(_dev(pos), _dev(S), n, ncrit, max(orders), theta)
+    gt.set_m2l_form("harmonic")
+    F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+    for order in orders:
+        _log(f"harmonic f64 {kind} n={n} ncrit={ncrit} p={order}")
+        Fo = O.OracleTree(pos, S, ncrit, order, theta).compute_field_fmm()
+        gt.set_order(order)
+        gt.compute_field_fmm(F, mode="plain")
+        err = util.rel_norm(F.cpu().numpy(), Fo)
+        _log(f"  vs oracle {err:.3e}")
+        assert err < TOL, order
+
+
+@pytest.mark.parametrize("order", [8, 12])
+def test_fmm_f32_full_form_matches_oracle(gpu, order):
+    """The FP32 GEMM on the full graded expansions (the form the automatic
+    choice replaces by the harmonic one) stays within 1e-6 of the oracle,
+    and the two forms agree to FP32 rounding."""
+    from fmmgen_amd import build_tree
+    n = 20000
+    pos = _positions("blobs", n, 23)
+    S = np.random.default_rng(24).uniform(-1, 1, (n, 1))
+    Fo = O.OracleTree(pos, S, 64, order, 0.5).compute_field_fmm()
+    gt = build_tree(_dev(pos), _dev(S), n, 64, order, 0.5)
+    out = {}
+    for form in ("full", "harmonic"):
+        _log(f"f32 {form} blobs n={n} p={order}")
+        gt.set_m2l_form(form)
+        F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+        gt.compute_field_fmm(F, mode="f32")
+        out[form] = F.cpu().numpy()
+        err = util.rel_norm(out[form][:, 1:], Fo[:, 1:])
+        _log(f"  vs oracle {err:.3e}")
+        assert err < 1e-6, form
+    assert util.rel_norm(out["full"][:, 1:], out["harmonic"][:, 1:]) < 1e-6
+
+
 # ------------------------------------------------------------------ other variants
 
 VARIANT_CASES = [
