@@ -414,8 +414,6 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   __shared__ unsigned sidx[NWORDS];  // offsets from each step's base slot, four steps a word
   __shared__ T sD[MD + 256];                 // slots MD .. MD + 255 stay zero
   __shared__ T sB[kTile * MSP + 4];          // tail slots stay zero
-  __shared__ signed char sdeg[MD];           // degree of each D slot
-  __shared__ signed char sdegB[LAY == kHarmonic && SCALED ? MS : 1];  // degree of each M slot
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int ci = xcd_tile(blockIdx.x, nblocks);
   if (ci >= nchunks) return;  // uniform over the block
@@ -424,9 +422,6 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   for (int i = tid; i < NWORDS; i += 64 * W) sidx[i] = idxtab[i];
   for (int i = tid; i < 256; i += 64 * W) sD[MD + i] = T(0);
   for (int i = tid; i < kTile * MSP + 4; i += 64 * W) sB[i] = T(0);
-  for (int j = tid; j < MD; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
-  if constexpr (LAY == kHarmonic && SCALED)
-    for (int j = tid; j < MS; j += 64 * W) sdegB[j] = (signed char)harm_degree(j);
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
   const int q = tile_order ? (int)tile_order[tile] : P;
@@ -441,6 +436,18 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 #pragma unroll
     for (int i = 0; i < (NIW > 0 ? NIW : 1); ++i) acc[i] = V4{0, 0, 0, 0};
     T rD[DJ], rB[CPW][NJ];
+    // degrees of the D and M slots this lane stages (fixed per lane)
+    int degD[DJ], degB[NJ];
+#pragma unroll
+    for (int i = 0; i < DJ; ++i) {
+      const int j = 64 * (wv + W * i) + lane;
+      degD[i] = j < MD ? slot_degree(j + Plan::OFF) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const int j = 64 * i + lane;
+      degB[i] = j < MS ? Plan::col_degree(j) : 0;
+    }
     // loads are issued unconditionally (empty columns read cell 0, slots
     // past the row read its last slot) and masked afterwards: no branch
     // around a load
@@ -471,7 +478,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        if (j < MD) sD[j] = sdeg[j] <= qc ? rD[i] : T(0);
+        if (j < MD) sD[j] = degD[i] <= qc ? rD[i] : T(0);
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c)
@@ -480,7 +487,7 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
           const int j = 64 * i + lane;
           if (j < MS) {
             T v = rB[c][i];
-            if (SCALED && dl != 0) v = ldexpf(v, dl * (LAY == kHarmonic ? sdegB[j] : sdeg[j]));
+            if constexpr (SCALED) v = ldexpf(v, __mul24(dl, degB[i]));  // exact: a power of two
             sB[(wv + W * c) * MSP + j] = v;
           }
         }
