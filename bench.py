@@ -68,11 +68,9 @@ def mac_flops_m2l(p, so, harmonic=False):
 
 def harmonic_m2l(form, mode, so):
     """Whether the M2L of this mode runs in the harmonic form."""
-    if so != 0 or form == "full" or mode in ("bh",):
+    if so != 0 or form == "full" or mode == "bh":
         return False
-    if form == "harmonic":
-        return mode not in ("bf16x3", "learned_bf16x3")
-    return mode in ("f32", "learned")
+    return form == "harmonic" or mode != "plain"
 
 
 def cpu_baseline(cfg):
@@ -212,7 +210,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--model", default=None)
     ap.add_argument("--m2l-form", default="auto", choices=["auto", "full", "harmonic"],
-                    help="M2L operand form of the timed mode (auto: harmonic for f32 / learned)")
+                    help="M2L operand form of the timed mode (auto: harmonic for the mixed modes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -104,7 +104,7 @@ constexpr int kTile = 16;        // target cells per M2L tile (one MFMA column b
 // Operand form of the M2L GEMM: the full graded expansions (the generated
 // operator's sums), or the harmonic layout (folded multipoles, local rows
 // with n_z <= 1, the same sums by the harmonicity of 1/R); automatic = the
-// harmonic layout for the FP32 GEMM, the full form for FP64 and BF16x3.
+// harmonic layout for the FP32 and BF16x3 GEMMs, the full form for FP64.
 enum : int { kM2LAuto = 0, kM2LFull = 1, kM2LHarmonic = 2 };
 
 struct M2LSchedule {
@@ -126,8 +126,8 @@ struct M2LSchedule {
   // order and block size (4 or 8 waves)
   static constexpr int kMaxOrder = 16;
   DevBuf<unsigned> idxtab[2][kMaxOrder][2];  // [layout][order][4 or 8 waves]
-  DevBuf<unsigned> idxtab32[kMaxOrder];
-  bool idx_ready[2][kMaxOrder][2] = {}, idx32_ready[kMaxOrder] = {false};
+  DevBuf<unsigned> idxtab32[2][kMaxOrder];  // [layout][order]
+  bool idx_ready[2][kMaxOrder][2] = {}, idx32_ready[2][kMaxOrder] = {};
   // learned order: chunks sorted by their tile's order, and the start of
   // every order's range (kMaxOrder + 1 entries)
   DevBuf<int> class_chunks;

@@ -128,8 +128,8 @@ struct StepPlan {
 };
 template <int P, int SO, int LAY = kGraded>
 using M2LPlan = StepPlan<P, SO, 4, LAY>;  // FP32 / FP64 16x16x4 MFMAs
-template <int P, int SO>
-using M2LPlan32 = StepPlan<P, SO, 32>;  // BF16 16x16x32 MFMAs
+template <int P, int SO, int LAY = kGraded>
+using M2LPlan32 = StepPlan<P, SO, 32, LAY>;  // BF16 16x16x32 MFMAs
 
 // Multi-index of slot j of the graded layout and its inverse:
 // idx(n) = Nterms(|n| - 1) + T(|n| - n_x) + (|n| - n_x - n_y), T(a) = a(a+1)/2.
@@ -341,17 +341,17 @@ struct WavePlanRR {
 template <int P, int SO, int W, int WV, int LAY = kGraded>
 using WavePlanFor = std::conditional_t<(P >= 6), WavePlan<M2LPlan<P, SO, LAY>, W, WV>,
                                        WavePlanRR<M2LPlan<P, SO, LAY>, W, WV>>;
-template <int P, int SO, int W, int WV>
-using WavePlan32For = WavePlan<M2LPlan32<P, SO>, W, WV>;
+template <int P, int SO, int W, int WV, int LAY = kGraded>
+using WavePlan32For = WavePlan<M2LPlan32<P, SO, LAY>, W, WV>;
 
 // Steps before wave wv in the BF16 plan's index table (4 words per lane
 // and step: eight 16-bit D slots).
-template <int P, int SO, int W, int WV>
+template <int P, int SO, int W, int WV, int LAY = kGraded>
 constexpr int wave_steps32() {
   if constexpr (WV == 0) {
     return 0;
   } else {
-    return wave_steps32<P, SO, W, WV - 1>() + WavePlan32For<P, SO, W, WV - 1>::tab.n;
+    return wave_steps32<P, SO, W, WV - 1, LAY>() + WavePlan32For<P, SO, W, WV - 1, LAY>::tab.n;
   }
 }
 
@@ -648,12 +648,12 @@ __device__ __forceinline__ void split_bf16(const float (&v)[8], bf16x8& hi, bf16
 
 // Waves per block of the BF16 kernel: a wave's index words (4 per step)
 // stay within about 40 registers.
-template <int P, int SO>
+template <int P, int SO, int LAY = kGraded>
 constexpr int m2l_waves32() {
-  return M2LPlan32<P, SO>::NSTEP > 40 ? 8 : 4;
+  return M2LPlan32<P, SO, LAY>::NSTEP > 40 ? 8 : 4;
 }
 
-template <int P, int SO, int W>
+template <int P, int SO, int W, int LAY = kGraded>
 __global__ __launch_bounds__(64 * W) void k_m2l_bf16x3(int nchunks, int nblocks, const int* __restrict__ tile_cells,
                                                        const int* __restrict__ chunk_tile,
                                                        const int* __restrict__ chunk_g0,
@@ -664,26 +664,26 @@ __global__ __launch_bounds__(64 * W) void k_m2l_bf16x3(int nchunks, int nblocks,
                                                        const float* __restrict__ Dtab,
                                                        const unsigned* __restrict__ idxtab,
                                                        const signed char* __restrict__ tile_order,
-                                                       const int* __restrict__ chunk_list, int mstride, int lstride,
-                                                       const float* __restrict__ M, float* __restrict__ L) {
-  using Plan = M2LPlan32<P, SO>;
+                                                       const int* __restrict__ chunk_list, int dstride,
+                                                       int mstride, int lstride, const float* __restrict__ M,
+                                                       float* __restrict__ L) {
+  using Plan = M2LPlan32<P, SO, LAY>;
   constexpr int MS = Plan::MS, LS = Plan::LS, NK = Plan::NK;
+  constexpr int MD = msize(P, SO);                 // D vector slots
   constexpr int RS = ((32 * NK + 3) / 4) * 4 + 4;  // row stride: 16-byte aligned reads, zero tail
   constexpr int NJ = (MS + 63) / 64;
   constexpr int CPW = kTile / W;
-  constexpr int DJ = (NJ + W - 1) / W;
+  constexpr int DJ = ((MD + 63) / 64 + W - 1) / W;
   using V4 = typename Vec4<float>::type;
-  __shared__ unsigned sD[MS + 1];  // D split once per group: packed (hi | lo << 16); slot MS stays zero
+  __shared__ unsigned sD[MD + 1];  // D split once per group: packed (hi | lo << 16); slot MD stays zero
   __shared__ __attribute__((aligned(16))) float sB[kTile * RS];
-  __shared__ signed char sdeg[MS];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int ci = xcd_tile(blockIdx.x, nblocks);
   if (ci >= nchunks) return;
   const int chunk = chunk_list ? chunk_list[ci] : ci;
   const int tile = chunk_tile[chunk];
-  if (tid == 0) sD[MS] = 0u;
+  if (tid == 0) sD[MD] = 0u;
   for (int i = tid; i < kTile * RS; i += 64 * W) sB[i] = 0.f;
-  for (int j = tid; j < MS; j += 64 * W) sdeg[j] = (signed char)slot_degree(j + Plan::OFF);
   const int col = lane & 15, kk = lane >> 4;
   const int g0 = chunk_g0[chunk], g1 = chunk_g1[chunk];
   const int q = tile_order ? (int)tile_order[tile] : P;
@@ -691,10 +691,10 @@ __global__ __launch_bounds__(64 * W) void k_m2l_bf16x3(int nchunks, int nblocks,
 
   dispatch_wave<W>(w, [&](auto WV) {
     constexpr int wv = WV;
-    using WP = WavePlan32For<P, SO, W, wv>;
+    using WP = WavePlan32For<P, SO, W, wv, LAY>;
     constexpr int NIW = WP::tab.nrow;
     constexpr int NSW = WP::tab.n;
-    constexpr int WS = wave_steps32<P, SO, W, wv>();
+    constexpr int WS = wave_steps32<P, SO, W, wv, LAY>();
     u32x4 ix[NSW > 0 ? NSW : 1];
     static_for<NSW>([&](auto i) {
       ix[i] = *reinterpret_cast<const u32x4*>(idxtab + ((size_t)(WS + i) * 64 + lane) * 4);
@@ -703,14 +703,25 @@ __global__ __launch_bounds__(64 * W) void k_m2l_bf16x3(int nchunks, int nblocks,
 #pragma unroll
     for (int i = 0; i < (NIW > 0 ? NIW : 1); ++i) acc[i] = V4{0, 0, 0, 0};
     float rD[DJ], rB[CPW][NJ];
+    int degD[DJ], degB[NJ];  // degrees of the D and M slots this lane stages
+#pragma unroll
+    for (int i = 0; i < DJ; ++i) {
+      const int j = 64 * (wv + W * i) + lane;
+      degD[i] = j < MD ? slot_degree(j + Plan::OFF) : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < NJ; ++i) {
+      const int j = 64 * i + lane;
+      degB[i] = j < MS ? Plan::col_degree(j) : 0;
+    }
     // loads are issued unconditionally and masked afterwards
     auto prefetch = [&](int g) {
       const int key = group_key[g];
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        const float v = Dtab[(size_t)key * mstride + (j < MS ? j : MS - 1)];
-        rD[i] = j < MS ? v : 0.f;
+        const float v = Dtab[(size_t)key * dstride + (j < MD ? j : MD - 1)];
+        rD[i] = j < MD ? v : 0.f;
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c) {
@@ -731,14 +742,14 @@ __global__ __launch_bounds__(64 * W) void k_m2l_bf16x3(int nchunks, int nblocks,
 #pragma unroll
       for (int i = 0; i < DJ; ++i) {
         const int j = 64 * (wv + W * i) + lane;
-        if (j < MS) sD[j] = sdeg[j] <= qc ? pack_split(rD[i]) : 0u;
+        if (j < MD) sD[j] = degD[i] <= qc ? pack_split(rD[i]) : 0u;
       }
 #pragma unroll
       for (int c = 0; c < CPW; ++c)
 #pragma unroll
         for (int i = 0; i < NJ; ++i) {
           const int j = 64 * i + lane;
-          if (j < MS) sB[(wv + W * c) * RS + j] = dl != 0 ? ldexpf(rB[c][i], dl * sdeg[j]) : rB[c][i];
+          if (j < MS) sB[(wv + W * c) * RS + j] = ldexpf(rB[c][i], __mul24(dl, degB[i]));  // exact
         }
       __syncthreads();  // staged operands visible to every wave
       if (g + 1 < g1) prefetch(g + 1);  // in flight during the MFMAs below
@@ -784,50 +795,38 @@ __global__ __launch_bounds__(64 * W) void k_m2l_bf16x3(int nchunks, int nblocks,
 
 // Host side: the BF16 plan's table for a W-wave block, per wave step (wave
 // order) and lane four words holding the eight 16-bit D slots
-// A[l%16][8 (l/16) + i] reads (MS, a zero slot, for an empty entry).
-template <int P, int SO, int W, int WV>
+// A[l%16][8 (l/16) + i] reads (msize(P, SO), a zero slot, for an empty
+// entry); rows and columns in the layout LAY, D graded.
+template <int P, int SO, int W, int WV, int LAY>
 void fill_wave_index32(std::vector<unsigned>& t) {
   if constexpr (WV < W) {
-    using WP = WavePlan32For<P, SO, W, WV>;
-    using Plan = M2LPlan32<P, SO>;
-    std::vector<int> mx, my, mz;
-    for (int d = 0; d <= 2 * P; ++d)
-      for (int x = d; x >= 0; --x)
-        for (int y = d - x; y >= 0; --y) {
-          mx.push_back(x);
-          my.push_back(y);
-          mz.push_back(d - x - y);
-        }
-    auto idx = [](int x, int y, int z) {
-      const int d = x + y + z;
-      const int a = d - x;
-      return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
-    };
-    const size_t s0 = (size_t)wave_steps32<P, SO, W, WV>();
+    using WP = WavePlan32For<P, SO, W, WV, LAY>;
+    using Plan = M2LPlan32<P, SO, LAY>;
+    const size_t s0 = (size_t)wave_steps32<P, SO, W, WV, LAY>();
     for (int i = 0; i < WP::tab.n; ++i) {
       const int st = WP::tab.s[i];
       const int I = Plan::tab.sI[st], K = Plan::tab.sK[st];
       for (int l = 0; l < 64; ++l)
         for (int e = 0; e < 8; ++e) {
           const int n = 16 * I + (l & 15), m = 32 * K + 8 * (l >> 4) + e;
-          unsigned u = (unsigned)Plan::MS;
+          unsigned u = (unsigned)msize(P, SO);
           if (n < Plan::LS && m < Plan::MS) {
-            const int mf = m + Plan::OFF;
-            if (mx[n] + my[n] + mz[n] + mx[mf] + my[mf] + mz[mf] <= P)
-              u = (unsigned)(idx(mx[n] + mx[mf], my[n] + my[mf], mz[n] + mz[mf]) - Plan::OFF);
+            const MultiIdx a = lay_multi<LAY>(n, 0), b = lay_multi<LAY>(m, Plan::OFF);
+            if (a.x + a.y + a.z + b.x + b.y + b.z <= P)
+              u = (unsigned)(multi_slot(a.x + b.x, a.y + b.y, a.z + b.z) - Plan::OFF);
           }
           unsigned& word = t[((s0 + i) * 64 + l) * 4 + e / 2];
           word = (e % 2 == 0) ? ((word & 0xffff0000u) | u) : ((word & 0xffffu) | (u << 16));
         }
     }
-    fill_wave_index32<P, SO, W, WV + 1>(t);
+    fill_wave_index32<P, SO, W, WV + 1, LAY>(t);
   }
 }
 
-template <int P, int SO, int W>
+template <int P, int SO, int W, int LAY = kGraded>
 std::vector<unsigned> m2l_index_table32() {
-  std::vector<unsigned> t((size_t)M2LPlan32<P, SO>::NSTEP * 64 * 4, 0u);
-  fill_wave_index32<P, SO, W, 0>(t);
+  std::vector<unsigned> t((size_t)M2LPlan32<P, SO, LAY>::NSTEP * 64 * 4, 0u);
+  fill_wave_index32<P, SO, W, 0, LAY>(t);
   return t;
 }
 

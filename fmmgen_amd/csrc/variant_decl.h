@@ -38,10 +38,8 @@ template <int Q>
 void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ps, hipStream_t st);
 // Whether the M2L of an evaluation in this mode runs in the harmonic layout.
 inline bool m2l_harmonic(const Tree& t, int mode) {
-  if (SO != 0) return false;
-  if (t.m2l_form == kM2LFull) return false;
-  if (t.m2l_form == kM2LHarmonic) return mode != 2;
-  return mode == 1;  // automatic: the FP32 GEMM of the mixed and learned modes
+  if (SO != 0 || t.m2l_form == kM2LFull) return false;
+  return t.m2l_form == kM2LHarmonic || mode != 0;  // automatic: the mixed and learned modes
 }
 
 // Order-independent phases (defined in the base translation unit).

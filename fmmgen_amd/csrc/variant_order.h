@@ -150,18 +150,18 @@ void harm_unfold(Tree& t, const DevBuf<T>& Lh, T* L, int ls, hipStream_t st) {
   k_harm_unfold<PS, T><<<(int)((n + 255) / 256), 256, 0, st>>>(t.ncells, Lh.get(), harm_size(PS), L, ls);
 }
 
-template <int P, int W>
+template <int P, int W, int LAY = kGraded>
 const unsigned* ensure_index_table32(Tree& t, hipStream_t st) {
   M2LSchedule& S = t.sched;
-  if (!S.idx32_ready[P]) {
-    const std::vector<unsigned> tab = m2l_index_table32<P, SO, W>();
-    S.idxtab32[P].alloc(tab.size());
-    FMM_HIP(hipMemcpyAsync(S.idxtab32[P].get(), tab.data(), sizeof(unsigned) * tab.size(), hipMemcpyHostToDevice,
-                           st));
+  if (!S.idx32_ready[LAY][P]) {
+    const std::vector<unsigned> tab = m2l_index_table32<P, SO, W, LAY>();
+    S.idxtab32[LAY][P].alloc(tab.size());
+    FMM_HIP(hipMemcpyAsync(S.idxtab32[LAY][P].get(), tab.data(), sizeof(unsigned) * tab.size(),
+                           hipMemcpyHostToDevice, st));
     FMM_HIP(hipStreamSynchronize(st));
-    S.idx32_ready[P] = true;
+    S.idx32_ready[LAY][P] = true;
   }
-  return S.idxtab32[P].get();
+  return S.idxtab32[LAY][P].get();
 }
 
 template <int P>
@@ -289,12 +289,23 @@ void m2l_class(Tree& t, int mode, const int* chunks, int nch, int ps, hipStream_
   const int grid = ((nch + 7) / 8) * 8;
   const int ms = msize(ps, SO), ls = lsize(ps, SO);
   if (mode == 2) {  // BF16x3 split precision
+    if constexpr (SO == 0) {
+      if (m2l_harmonic(t, mode)) {
+        constexpr int W32 = m2l_waves32<Q, SO, kHarmonic>();
+        const unsigned* idx = ensure_index_table32<Q, W32, kHarmonic>(t, st);
+        k_m2l_bf16x3<Q, SO, W32, kHarmonic><<<grid, 64 * W32, 0, st>>>(
+            nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(), S.chunk_g1.get(), S.group_key.get(),
+            S.group_src.get(), S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks, ms,
+            harm_size(ps), harm_size(ps), t.Mhf.get(), t.Lhf.get());
+        return;
+      }
+    }
     constexpr int W32 = m2l_waves32<Q, SO>();
     const unsigned* idx = ensure_index_table32<Q, W32>(t, st);
     k_m2l_bf16x3<Q, SO, W32><<<grid, 64 * W32, 0, st>>>(nch, grid, S.tile_cells.get(), S.chunk_tile.get(), S.chunk_g0.get(),
                                              S.chunk_g1.get(), S.group_key.get(), S.group_src.get(),
                                              S.key_dlevel.get(), S.Dtab32.get(), idx, t.tile_order_ptr(), chunks,
-                                             ms, ls, t.Mf.get(), t.Lf.get());
+                                             ms, ms, ls, t.Mf.get(), t.Lf.get());
     return;
   }
   if constexpr (SO == 0) {

@@ -64,12 +64,12 @@ class Tree:
             self._h = None
 
     def set_m2l_form(self, form):
-        """Operand form of the M2L translation: "auto" (harmonic for the FP32
-        GEMM of the f32 and learned modes, full otherwise), "full" (the
-        generated operator's sums over every multi-index) or "harmonic"
-        (folded multipoles and the local rows with n_z <= 1: the same
-        expansions by the harmonicity of 1/R, (p+1)^2 rows and columns
-        instead of C(p+3,3); the BF16x3 modes keep the full form)."""
+        """Operand form of the M2L translation: "auto" (harmonic for the
+        mixed and learned modes, full for plain FP64), "full" (the generated
+        operator's sums over every multi-index) or "harmonic" (folded
+        multipoles and the local rows with n_z <= 1: the same expansions by
+        the harmonicity of 1/R, (p+1)^2 rows and columns instead of
+        C(p+3,3)).  Charges only; other source orders use the full form."""
         if form not in _lib.M2L_FORMS:
             raise ValueError(f"M2L form must be one of {sorted(_lib.M2L_FORMS)}")
         _lib.check(_lib.load().fmm_tree_set_m2l_form(self._h, _lib.M2L_FORMS[form]))

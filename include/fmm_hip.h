@@ -204,13 +204,13 @@ int fmm_mlp_adam(const float* d_grad, float* d_params, float* d_m, float* d_v, f
 int fmm_tree_set_order(fmm_tree_t tree, int order);
 /* Operand form of the M2L translation (no counterpart in the reference,
  * whose M2L is the generated operator, fmmgen/expansions.py:144-172):
- *   FMM_M2L_AUTO (default): harmonic for the FP32 GEMM (the f32 and learned
- *     modes), full for plain FP64 and the BF16x3 modes;
+ *   FMM_M2L_AUTO (default): harmonic for the mixed and learned modes, full
+ *     for plain FP64;
  *   FMM_M2L_FULL: the generated operator's sums over every multi-index;
  *   FMM_M2L_HARMONIC: the same local expansions from folded multipoles and
  *     the rows with n_z <= 1 only ((p+1)^2 instead of C(p+3,3) per side,
  *     exact by the harmonicity of 1/R; results differ from the full form by
- *     rounding); the BF16x3 modes keep the full form.
+ *     rounding), in every mode.
  * Charges only (source order 0): other variants always use the full form. */
 #define FMM_M2L_AUTO 0
 #define FMM_M2L_FULL 1

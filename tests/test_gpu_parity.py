@@ -281,11 +281,11 @@ def test_fmm_harmonic_m2l_f64_matches_oracle(gpu, kind, n, ncrit, orders, theta)
         assert err < TOL, order
 
 
-@pytest.mark.parametrize("order", [8, 12])
-def test_fmm_f32_full_form_matches_oracle(gpu, order):
-    """The FP32 GEMM on the full graded expansions (the form the automatic
-    choice replaces by the harmonic one) stays within 1e-6 of the oracle,
-    and the two forms agree to FP32 rounding."""
+@pytest.mark.parametrize("order,mode", [(8, "f32"), (12, "f32"), (8, "bf16x3"), (12, "bf16x3")])
+def test_fmm_mixed_full_form_matches_oracle(gpu, order, mode):
+    """The FP32 and BF16x3 GEMMs on the full graded expansions (the form the
+    automatic choice replaces by the harmonic one) stay within 1e-6 of the
+    oracle, and the two forms agree to FP32 rounding."""
     from fmmgen_amd import build_tree
     n = 20000
     pos = _positions("blobs", n, 23)
@@ -294,10 +294,10 @@ def test_fmm_f32_full_form_matches_oracle(gpu, order):
     gt = build_tree(_dev(pos), _dev(S), n, 64, order, 0.5)
     out = {}
     for form in ("full", "harmonic"):
-        _log(f"f32 {form} blobs n={n} p={order}")
+        _log(f"{mode} {form} blobs n={n} p={order}")
         gt.set_m2l_form(form)
         F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
-        gt.compute_field_fmm(F, mode="f32")
+        gt.compute_field_fmm(F, mode=mode)
         out[form] = F.cpu().numpy()
         err = util.rel_norm(out[form][:, 1:], Fo[:, 1:])
         _log(f"  vs oracle {err:.3e}")
