@@ -546,60 +546,138 @@ __global__ __launch_bounds__(64 * W) void k_m2l_gemm(int nchunks, int nblocks, c
   });
 }
 
-// Pascal's triangle up to row 15 (the fold and unfold coefficients).
-struct Binom {
-  int c[16][16];
-  constexpr Binom() : c{} {
-    for (int n = 0; n < 16; ++n) {
-      c[n][0] = 1;
-      for (int k = 1; k <= n; ++k) c[n][k] = c[n - 1][k - 1] + (k < n ? c[n - 1][k] : 0);
+// Fold and unfold as term tables (see the harmonic layout above): for each
+// harmonic slot j, the graded multipole slots that fold onto it and their
+// weights (-1)^k C(k,i); for each graded local slot n, the harmonic rows it
+// is rebuilt from.  Built at compile time per order.
+template <int P>
+struct HarmTables {
+  static constexpr int HS = harm_size(P), GS = nterms(P);
+  static constexpr int binom(int n, int k) {
+    int c = 1;
+    for (int i = 0; i < k; ++i) c = c * (n - i) / (i + 1);
+    return c;
+  }
+  static constexpr int count_fold() {
+    int t = 0;
+    for (int j = 0; j < HS; ++j) {
+      const MultiIdx h = harm_multi(j);
+      for (int k = 0; 2 * k <= h.x + h.y; ++k)
+        for (int a = 0; a <= k; ++a)
+          if (h.x - 2 * a >= 0 && h.y - 2 * (k - a) >= 0) ++t;
     }
+    return t;
+  }
+  static constexpr int count_unfold() {
+    int t = 0;
+    for (int n = 0; n < GS; ++n) t += slot_multi(n).z / 2 + 1;
+    return t;
+  }
+  static constexpr int NF = count_fold(), NU = count_unfold();
+  short fold_off[HS + 1], fold_src[NF];
+  signed char fold_w[NF];
+  short unf_off[GS + 1], unf_src[NU];
+  signed char unf_w[NU];
+  constexpr HarmTables() : fold_off{}, fold_src{}, fold_w{}, unf_off{}, unf_src{}, unf_w{} {
+    int t = 0;
+    for (int j = 0; j < HS; ++j) {
+      fold_off[j] = (short)t;
+      const MultiIdx h = harm_multi(j);
+      for (int k = 0; 2 * k <= h.x + h.y; ++k)
+        for (int a = 0; a <= k; ++a) {
+          const int sx = h.x - 2 * a, sy = h.y - 2 * (k - a);
+          if (sx >= 0 && sy >= 0) {
+            fold_src[t] = (short)multi_slot(sx, sy, h.z + 2 * k);
+            fold_w[t] = (signed char)((k & 1) ? -binom(k, a) : binom(k, a));
+            ++t;
+          }
+        }
+    }
+    fold_off[HS] = (short)t;
+    t = 0;
+    for (int n = 0; n < GS; ++n) {
+      unf_off[n] = (short)t;
+      const MultiIdx m = slot_multi(n);
+      const int k = m.z / 2, c = m.z % 2;
+      for (int a = 0; a <= k; ++a) {
+        unf_src[t] = (short)harm_slot(m.x + 2 * a, m.y + 2 * (k - a), c);
+        unf_w[t] = (signed char)((k & 1) ? -binom(k, a) : binom(k, a));
+        ++t;
+      }
+    }
+    unf_off[GS] = (short)t;
   }
 };
 
-// Fold (see the harmonic layout above): Mh[cell][j] for the (P + 1)^2
-// harmonic slots j from the graded multipole M[cell] (stride ms), one
-// thread per (cell, slot), summed in FP64.
+// Fold and unfold kernels: a block of 4 waves stages the term table in LDS
+// once, then each wave takes kHarmCellsPerWave cells in turn, copying the
+// cell's source row into LDS with coalesced loads and summing its terms (FP64)
+// from LDS.
+constexpr int kHarmCellsPerWave = 4;
+
+// Fold: Mh[cell][j] for the (P + 1)^2 harmonic slots j from the graded
+// multipole M[cell] (stride ms).
 template <int P, typename T>
 __global__ __launch_bounds__(256) void k_harm_fold(int nc, const T* __restrict__ M, int ms, T* __restrict__ Mh,
                                                    int hs) {
-  constexpr int HS = harm_size(P);
-  constexpr Binom B{};
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)nc * HS) return;
-  const int cell = (int)(i / HS), j = (int)(i % HS);
-  const MultiIdx h = harm_multi(j);
-  const T* m = M + (size_t)cell * ms;
-  double v = 0.0;
-  for (int k = 0; 2 * k <= h.x + h.y; ++k) {
-    double sk = 0.0;
-    for (int a = 0; a <= k; ++a) {  // source (x - 2a, y - 2(k - a), z + 2k)
-      const int sx = h.x - 2 * a, sy = h.y - 2 * (k - a);
-      if (sx >= 0 && sy >= 0) sk += (double)B.c[k][a] * (double)m[multi_slot(sx, sy, h.z + 2 * k)];
-    }
-    v += (k & 1) ? -sk : sk;
+  using H = HarmTables<P>;
+  constexpr H tab{};
+  __shared__ short s_off[H::HS + 1], s_src[H::NF];
+  __shared__ signed char s_w[H::NF];
+  __shared__ T s_row[4][H::GS];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  for (int i = tid; i <= H::HS; i += 256) s_off[i] = tab.fold_off[i];
+  for (int i = tid; i < H::NF; i += 256) {
+    s_src[i] = tab.fold_src[i];
+    s_w[i] = tab.fold_w[i];
   }
-  Mh[(size_t)cell * hs + j] = (T)v;
+  const int c0 = (blockIdx.x * 4 + w) * kHarmCellsPerWave;
+  for (int c = 0; c < kHarmCellsPerWave; ++c) {
+    const int cell = c0 + c;
+    __syncthreads();  // tables staged; the previous cell's row read
+    if (cell < nc)
+      for (int j = lane; j < H::GS; j += 64) s_row[w][j] = M[(size_t)cell * ms + j];
+    __syncthreads();
+    if (cell < nc)
+      for (int j = lane; j < H::HS; j += 64) {
+        double v = 0.0;
+        for (int t = s_off[j]; t < s_off[j + 1]; ++t) v += (double)s_w[t] * (double)s_row[w][s_src[t]];
+        Mh[(size_t)cell * hs + j] = (T)v;
+      }
+  }
 }
 
-// Unfold: L[cell][n] += the graded local expansion of the harmonic rows
-// Lh[cell] (stride hs), one thread per (cell, graded slot), in FP64.
+// Unfold: L[cell][n] += the graded local expansion rebuilt from the harmonic
+// rows Lh[cell] (stride hs).
 template <int P, typename T>
 __global__ __launch_bounds__(256) void k_harm_unfold(int nc, const T* __restrict__ Lh, int hs, T* __restrict__ L,
                                                      int ls) {
-  constexpr int LS = nterms(P);
-  constexpr Binom B{};
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (long long)nc * LS) return;
-  const int cell = (int)(i / LS), n = (int)(i % LS);
-  const MultiIdx t = slot_multi(n);
-  const T* h = Lh + (size_t)cell * hs;
-  const int k = t.z >> 1, c = t.z & 1;
-  double v = 0.0;
-  for (int a = 0; a <= k; ++a) v += (double)B.c[k][a] * (double)h[harm_slot(t.x + 2 * a, t.y + 2 * (k - a), c)];
-  if (k & 1) v = -v;
-  T* o = L + (size_t)cell * ls + n;
-  *o = (T)((double)*o + v);
+  using H = HarmTables<P>;
+  constexpr H tab{};
+  __shared__ short s_off[H::GS + 1], s_src[H::NU];
+  __shared__ signed char s_w[H::NU];
+  __shared__ T s_row[4][H::HS];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  for (int i = tid; i <= H::GS; i += 256) s_off[i] = tab.unf_off[i];
+  for (int i = tid; i < H::NU; i += 256) {
+    s_src[i] = tab.unf_src[i];
+    s_w[i] = tab.unf_w[i];
+  }
+  const int c0 = (blockIdx.x * 4 + w) * kHarmCellsPerWave;
+  for (int c = 0; c < kHarmCellsPerWave; ++c) {
+    const int cell = c0 + c;
+    __syncthreads();
+    if (cell < nc)
+      for (int j = lane; j < H::HS; j += 64) s_row[w][j] = Lh[(size_t)cell * hs + j];
+    __syncthreads();
+    if (cell < nc)
+      for (int n = lane; n < H::GS; n += 64) {
+        double v = 0.0;
+        for (int t = s_off[n]; t < s_off[n + 1]; ++t) v += (double)s_w[t] * (double)s_row[w][s_src[t]];
+        T* o = L + (size_t)cell * ls + n;
+        *o = (T)((double)*o + v);
+      }
+  }
 }
 
 // ---------------------------------------------------------------- BF16x3

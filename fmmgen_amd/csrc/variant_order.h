@@ -140,14 +140,16 @@ void harm_fold(Tree& t, const T* M, int ms, DevBuf<T>& Mh, DevBuf<T>& Lh, hipStr
   const int nc = t.ncells, hs = harm_size(PS);
   Mh.alloc((size_t)nc * hs);
   Lh.alloc((size_t)nc * hs);
-  const long long n = (long long)nc * hs;
-  k_harm_fold<PS, T><<<(int)((n + 255) / 256), 256, 0, st>>>(nc, M, ms, Mh.get(), hs);
-  FMM_HIP(hipMemsetAsync(Lh.get(), 0, sizeof(T) * n, st));
+  if (nc == 0) return;
+  constexpr int per = 4 * kHarmCellsPerWave;
+  k_harm_fold<PS, T><<<(nc + per - 1) / per, 256, 0, st>>>(nc, M, ms, Mh.get(), hs);
+  FMM_HIP(hipMemsetAsync(Lh.get(), 0, sizeof(T) * (size_t)nc * hs, st));
 }
 template <int PS, typename T>
 void harm_unfold(Tree& t, const DevBuf<T>& Lh, T* L, int ls, hipStream_t st) {
-  const long long n = (long long)t.ncells * nterms(PS);
-  k_harm_unfold<PS, T><<<(int)((n + 255) / 256), 256, 0, st>>>(t.ncells, Lh.get(), harm_size(PS), L, ls);
+  if (t.ncells == 0) return;
+  constexpr int per = 4 * kHarmCellsPerWave;
+  k_harm_unfold<PS, T><<<(t.ncells + per - 1) / per, 256, 0, st>>>(t.ncells, Lh.get(), harm_size(PS), L, ls);
 }
 
 template <int P, int W, int LAY = kGraded>
