@@ -69,6 +69,31 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
 // zero-filled Fd[J][OUT].
 constexpr int kJudgeSlice = 16384;
 constexpr int kJudgeTile = 8;
+
+// 1/sqrt(x) in FP64 for x > 0 (0 for x = 0): the hardware estimate y0
+// refined by the second-order series of (1 - e)^(-1/2), e = 1 - x y0^2:
+// y = y0 + y0 e (1/2 + 3e/8), which cubes the estimate's relative error.
+__device__ __forceinline__ double judge_rsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double e = fma(-x * y, y, 1.0);
+  const double r = fma(y * e, fma(0.375, e, 0.5), y);
+  return x > 0.0 ? r : 0.0;
+}
+
+// One charge pair of the judge (potential and field): the terms of the
+// example's P2P (example/calculate.cpp:24-40) with the reciprocal square
+// root above; a coincident pair (the target itself) adds nothing.
+template <int NOUT>
+__device__ __forceinline__ void judge_pair(double dx, double dy, double dz, double q, double (&acc)[NOUT]) {
+  const double ri = judge_rsqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
+  const double qr = q * ri;
+  const double qr3 = qr * ri * ri;
+  constexpr int F0 = NOUT == 4 ? 1 : 0;
+  if constexpr (NOUT == 4) acc[0] += qr;
+  acc[F0] = fma(qr3, dx, acc[F0]);
+  acc[F0 + 1] = fma(qr3, dy, acc[F0 + 1]);
+  acc[F0 + 2] = fma(qr3, dz, acc[F0 + 2]);
+}
 __global__ void __launch_bounds__(256) k_judge_direct(int n, const double* __restrict__ px,
                                                       const double* __restrict__ py, const double* __restrict__ pz,
                                                       const double* __restrict__ pS, const int* __restrict__ slots,
@@ -95,12 +120,16 @@ __global__ void __launch_bounds__(256) k_judge_direct(int n, const double* __res
 #pragma unroll
     for (int u = 0; u < kJudgeTile; ++u) {
       double dx = tx[u] - xj, dy = ty[u] - yj, dz = tz[u] - zj;
-      double s[SS];
-      const bool self = j == ti[u];
+      if constexpr (SO == 0 && (OUT == 4 || OUT == 3)) {
+        judge_pair<OUT>(dx, dy, dz, sj[0], acc[u]);  // r = 0 only for the target itself
+      } else {
+        double s[SS];
+        const bool self = j == ti[u];
 #pragma unroll
-      for (int k = 0; k < SS; ++k) s[k] = self ? 0.0 : sj[k];
-      if (self) dx = 1.0;  // finite distance, zero strength
-      fmmgen_ops::P2P<double>(dx, dy, dz, s, acc[u]);
+        for (int k = 0; k < SS; ++k) s[k] = self ? 0.0 : sj[k];
+        if (self) dx = 1.0;  // finite distance, zero strength
+        fmmgen_ops::P2P<double>(dx, dy, dz, s, acc[u]);
+      }
     }
   }
   // wave sums, block sums in LDS, one atomic per block and value
