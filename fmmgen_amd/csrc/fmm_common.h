@@ -125,9 +125,9 @@ struct M2LSchedule {
   // (step, lane) -> D slot tables of the GEMM plans, uploaded once per
   // order and block size (4 or 8 waves)
   static constexpr int kMaxOrder = 16;
-  DevBuf<unsigned> idxtab[2][kMaxOrder][2];  // [layout][order][4 or 8 waves]
+  DevBuf<unsigned> idxtab[2][kMaxOrder][3];  // [layout][order][4, 8 or 2 waves]
   DevBuf<unsigned> idxtab32[2][kMaxOrder];  // [layout][order]
-  bool idx_ready[2][kMaxOrder][2] = {}, idx32_ready[2][kMaxOrder] = {};
+  bool idx_ready[2][kMaxOrder][3] = {}, idx32_ready[2][kMaxOrder] = {};
   // learned order: chunks sorted by their tile's order, and the start of
   // every order's range (kMaxOrder + 1 entries)
   DevBuf<int> class_chunks;

@@ -373,6 +373,7 @@ template <int P, int SO, typename T, int LAY = kGraded>
 constexpr int m2l_waves() {
   using Plan = M2LPlan<P, SO, LAY>;
   constexpr int lds = Plan::NSTEP * 64 + (int)sizeof(T) * (kTile * (Plan::MS | 1) + msize(P, SO) + 256);
+  if (LAY == kHarmonic && P <= 8) return 2;  // few steps per group: fewer waves per barrier
   return lds > 80 * 1024 ? 8 : 4;
 }
 

@@ -118,9 +118,9 @@ __global__ void k_dtab(int nkeys, const double* __restrict__ key_dx, const doubl
 
 template <int P, int W, int LAY = kGraded>
 const unsigned* ensure_index_table(Tree& t, hipStream_t st) {
-  static_assert(W == 4 || W == 8, "blocks of 4 or 8 waves");
+  static_assert(W == 2 || W == 4 || W == 8, "blocks of 2, 4 or 8 waves");
   M2LSchedule& S = t.sched;
-  const int wi = W == 8 ? 1 : 0;
+  const int wi = W == 8 ? 1 : (W == 2 ? 2 : 0);
   if (!S.idx_ready[LAY][P][wi]) {
     const std::vector<unsigned> tab = m2l_index_table<P, SO, W, LAY>();
     S.idxtab[LAY][P][wi].alloc(tab.size());

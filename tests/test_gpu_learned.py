@@ -277,6 +277,37 @@ def test_learned_meets_tolerance_with_fewer_m2l_macs(gpu, kind, n, ncrit, fixed_
 
 
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
+@pytest.mark.parametrize("mode", ["learned", "learned_bf16x3"])
+def test_learned_full_form_m2l_meets_tolerance(gpu, mode):
+    """The learned modes with the M2L on the full graded expansions (the
+    per-order class launches of the full-form kernels) meet the tolerance
+    over every target and agree with the harmonic form to FP32 rounding."""
+    from fmmgen_amd import build_tree, direct_sum
+    from fmmgen_amd import learned as LN
+    tol, n = 1e-6, 1 << 16
+    _log(f"learned {mode} full form blobs n={n}")
+    pos = _positions("blobs", n, 31)
+    q = np.random.default_rng(32).uniform(-1, 1, (n, 1))
+    dpos, dq = _dev(pos), _dev(q)
+    tg = _all_targets(n)
+    Fd = direct_sum(dpos, dq, torch.from_numpy(tg).cuda()).cpu().numpy()
+    t = build_tree(dpos, dq, n, 64, 12, 0.5)
+    params, margin, _ = LN.load(MODEL)
+    t.set_m2l_form("full")
+    t.set_model(params, tol, margin)
+    out = {}
+    for form in ("full", "harmonic"):
+        t.set_m2l_form(form)
+        F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+        t.compute_field_fmm(F, mode=mode)
+        out[form] = F.cpu().numpy()
+        e, _ = _errors(out[form], Fd)
+        _log(f"  {form}: rel field error {e:.3e}, stats {t.learned_stats()}")
+        assert e <= tol, form
+    assert _rel_err(out["full"], out["harmonic"], np.arange(n)) < 1e-6
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
 def test_learned_full_size_blobs_meets_tolerance(gpu):
     """The headline workload at full size: N = 2^22 blobs, ncrit 64,
     theta 0.5, orders up to 12, tolerance 1e-6, checked at 4096 targets
