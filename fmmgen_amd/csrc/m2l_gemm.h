@@ -729,6 +729,7 @@ __device__ __forceinline__ void split_bf16(const float (&v)[8], bf16x8& hi, bf16
 // stay within about 40 registers.
 template <int P, int SO, int LAY = kGraded>
 constexpr int m2l_waves32() {
+  if (LAY == kHarmonic && P <= 8) return 2;  // few steps per group: fewer waves per barrier
   return M2LPlan32<P, SO, LAY>::NSTEP > 40 ? 8 : 4;
 }
 
