@@ -204,6 +204,8 @@ struct Tree {
   const int* tile_hint_ptr() const { return use_tile_hint ? tile_hint.get() : nullptr; }
   DevBuf<float> leaf_y;            // [nleaves * kMlpO] network outputs
   DevBuf<double> cell_qabs, cell_q2;  // [ncells] sum of |S| and of |S|^2 below the cell
+  DevBuf<double> cell_geo4;           // [ncells * 4] centre and rmax, packed for the feature gather
+  DevBuf<float> cell_qw;              // [ncells * 2] float(sum |S|), sqrt(float(sum S^2))
   DevBuf<double> cell_acc;            // [ncells * 11] list sums, accumulated down the tree
 
   DevBuf<signed char> tile_order;  // [ntiles]

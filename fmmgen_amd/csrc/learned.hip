@@ -63,25 +63,39 @@ __global__ void k_parent_max(int beg, int end, const int* __restrict__ first_chi
 // list sums of every target cell: acc[c] = (S0, S3, S6, S9, S12 with
 // w = sum|S| / R^2; T3, T6, T9, T12 with w = sqrt(sum |S|^2) / R^2; max rho;
 // count), S_k = sum w rho^k; one wavefront per cell, lanes over its list
+// Per-cell inputs of k_cell_lists packed for one gather per source cell:
+// geo[c] = (cx, cy, cz, rmax) in FP64, qw[c] = (float(sum |S|),
+// sqrtf(float(sum S^2))).
+__global__ void k_pack_cells(int ncells, const double* __restrict__ cx, const double* __restrict__ cy,
+                             const double* __restrict__ cz, const double* __restrict__ crmax,
+                             const double* __restrict__ qabs, const double* __restrict__ q2,
+                             double4* __restrict__ geo, float2* __restrict__ qw) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  geo[c] = make_double4(cx[c], cy[c], cz[c], crmax[c]);
+  qw[c] = make_float2((float)qabs[c], sqrtf((float)q2[c]));
+}
+
 __global__ __launch_bounds__(64) void k_cell_lists(int ncells, const int* __restrict__ off,
-                                                   const int* __restrict__ src, const double* __restrict__ cx,
-                                                   const double* __restrict__ cy, const double* __restrict__ cz,
-                                                   const double* __restrict__ crmax, const double* __restrict__ qabs,
-                                                   const double* __restrict__ q2, double* __restrict__ acc) {
+                                                   const int* __restrict__ src, const double4* __restrict__ geo,
+                                                   const float2* __restrict__ qw, double* __restrict__ acc) {
   const int A = blockIdx.x;
   if (A >= ncells) return;
   // features are estimates: per-pair arithmetic in FP32 (offsets from
   // FP64 differences), sums in FP64
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   float rmx = 0.f;
-  const double xa = cx[A], ya = cy[A], za = cz[A];
-  const float ra = (float)crmax[A];
+  const double4 ga = geo[A];
+  const double xa = ga.x, ya = ga.y, za = ga.z;
+  const float ra = (float)ga.w;
   for (int i = off[A] + threadIdx.x; i < off[A + 1]; i += 64) {
     const int B = src[i];
-    const float dx = (float)(xa - cx[B]), dy = (float)(ya - cy[B]), dz = (float)(za - cz[B]);
+    const double4 gb = geo[B];
+    const float2 qb = qw[B];
+    const float dx = (float)(xa - gb.x), dy = (float)(ya - gb.y), dz = (float)(za - gb.z);
     const float R2 = dx * dx + dy * dy + dz * dz;
-    const float rho = (ra + (float)crmax[B]) * __frsqrt_rn(R2);
-    const float w = (float)qabs[B] / R2, u = sqrtf((float)q2[B]) / R2;
+    const float rho = (ra + (float)gb.w) * __frsqrt_rn(R2);
+    const float w = qb.x / R2, u = qb.y / R2;
     const float r3 = rho * rho * rho, r6 = r3 * r3, r9 = r6 * r3, r12 = r6 * r6;
     v[0] += w;
     v[1] += w * r3;
@@ -359,37 +373,47 @@ __global__ void k_alloc_multi(int nl, const float* __restrict__ feat, const floa
                               const double* __restrict__ lw, float margin, int qmin, int qmax,
                               double* __restrict__ sums, int gated) {
   __shared__ float sh[kLam];
+  __shared__ double lamk[kLam];
   if (lw[kLwSkip] != 0.0 || (gated && lw[kLwRealloc] == 0.0)) return;  // uniform over the grid
   const double lo = lw[kLwLo], step = lw[kLwStep];
-  for (int k = threadIdx.x; k < kLam; k += blockDim.x) sh[k] = 0.f;
+  for (int k = threadIdx.x; k < kLam; k += blockDim.x) {
+    sh[k] = 0.f;
+    lamk[k] = exp10(lo + k * step);
+  }
   __syncthreads();
   const int li = blockIdx.x * blockDim.x + threadIdx.x;
-  if (li < nl) {
+  const bool act = li < nl;
+  double n = 0.0, c[kMlpO], e2[kMlpO];
+  for (int o = 0; o < kMlpO; ++o) c[o] = -1.0, e2[o] = 0.0;
+  if (act) {
     const float* f = feat + (size_t)li * kMlpF;
     const double w = fmax(exp10((double)f[5]) - 1.0, 1.0);
-    const double n = exp2((double)f[6]);
-    double c[kMlpO], e2[kMlpO];
+    n = exp2((double)f[6]);
     for (int o = 0; o < kMlpO; ++o) {
       const int q = o + kMlpOrder0;
       const bool ok = q >= qmin && q <= qmax;
       c[o] = ok ? macs(q) * w : -1.0;
       e2[o] = exp10(2.0 * ((double)Y[(size_t)li * kMlpO + o] + margin));
     }
-    for (int k = 0; k < kLam; ++k) {
-      const double lam = exp10(lo + k * step) * n;
-      double best = 0.0, be = 0.0;
-      bool any = false;
-      for (int o = 0; o < kMlpO; ++o) {
-        if (c[o] < 0.0) continue;
-        const double v = c[o] + lam * e2[o];
-        if (!any || v < best) {
-          best = v;
-          be = e2[o];
-          any = true;
-        }
+  }
+  // every lane of the wave runs all kLam multipliers: the wave's sum of each
+  // multiplier's n_l e_l^2 is one LDS atomic
+  for (int k = 0; k < kLam; ++k) {
+    const double lam = lamk[k] * n;
+    double best = 0.0, be = 0.0;
+    bool any = false;
+    for (int o = 0; o < kMlpO; ++o) {
+      if (c[o] < 0.0) continue;
+      const double v = c[o] + lam * e2[o];
+      if (!any || v < best) {
+        best = v;
+        be = e2[o];
+        any = true;
       }
-      atomicAdd(&sh[k], (float)(n * be));
     }
+    float contrib = act ? (float)(n * be) : 0.f;
+    for (int m = 32; m > 0; m >>= 1) contrib += __shfl_xor(contrib, m, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&sh[k], contrib);
   }
   __syncthreads();
   for (int k = threadIdx.x; k < kLam; k += blockDim.x) atomicAdd(&sums[k], (double)sh[k]);
@@ -667,8 +691,13 @@ void leaf_features(Tree& t, int ss, hipStream_t st) {
     k_parent_sum<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cell_qabs.get());
     k_parent_sum<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cell_q2.get());
   }
-  k_cell_lists<<<nc, 64, 0, st>>>(nc, t.m2l_off.get(), t.m2l_src.get(), t.cx.get(), t.cy.get(), t.cz.get(),
-                                           t.crmax.get(), t.cell_qabs.get(), t.cell_q2.get(), t.cell_acc.get());
+  t.cell_geo4.alloc((size_t)nc * 4);
+  t.cell_qw.alloc((size_t)nc * 2);
+  double4* geo = reinterpret_cast<double4*>(t.cell_geo4.get());
+  float2* qw = reinterpret_cast<float2*>(t.cell_qw.get());
+  k_pack_cells<<<nb(nc, 256), 256, 0, st>>>(nc, t.cx.get(), t.cy.get(), t.cz.get(), t.crmax.get(),
+                                           t.cell_qabs.get(), t.cell_q2.get(), geo, qw);
+  k_cell_lists<<<nc, 64, 0, st>>>(nc, t.m2l_off.get(), t.m2l_src.get(), geo, qw, t.cell_acc.get());
   for (int l = 1; l < t.depth; ++l) {
     const int b = t.level_off[l], e = t.level_off[l + 1];
     k_acc_down<<<nb(e - b, 256), 256, 0, st>>>(b, e, t.parent.get(), t.cell_acc.get());

@@ -302,6 +302,7 @@ __global__ __launch_bounds__(512) void k_m2m_top_wave(int beg, int end, const in
   static_assert(SO == 0 || P < 0, "charges only");
   constexpr int MS = Op<P>::MS, NPL = (MS + 63) / 64;
   __shared__ double sM[8][MS];  // wave w: child w's scaled multipole, then its contribution
+  __shared__ double sT[8][MS];  // the x- and z-pass results
   __shared__ double pw[8][3][P + 1];
   const int p = beg + blockIdx.x;
   if (p >= end) return;
@@ -325,30 +326,45 @@ __global__ __launch_bounds__(512) void k_m2m_top_wave(int beg, int end, const in
       pw[w][d][a] = v;
     }
   }
-  __syncthreads();
-  if (w < k) {
+  // the shift M'_n = sum_{j <= n} M_j u^(n-j)/(n-j)! factors over the three
+  // axes: one pass per axis, each a sum of at most P + 1 terms per slot
+  MultiIdx mi[NPL];
 #pragma unroll
-    for (int i = 0; i < NPL; ++i) {
-      const int n = lane + 64 * i;
-      if (n < MS) {
-        const MultiIdx m = slot_multi(n);
-        double sum = 0.0;
-        for (int ia = 0; ia <= m.x; ++ia)
-          for (int jb = 0; jb <= m.y; ++jb) {
-            const double xy = pw[w][0][ia] * pw[w][1][jb];
-            for (int kc = 0; kc <= m.z; ++kc)
-              sum += sM[w][multi_slot(m.x - ia, m.y - jb, m.z - kc)] * (xy * pw[w][2][kc]);
-          }
-        acc[i] = sum;
+  for (int i = 0; i < NPL; ++i) mi[i] = slot_multi(min(lane + 64 * i, MS - 1));
+  __syncthreads();
+  auto pass = [&](const double* in, double* out, int d) {
+    if (w < k) {
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        const int n = lane + 64 * i;
+        if (n < MS) {
+          const MultiIdx m = mi[i];
+          const int e = d == 0 ? m.x : (d == 1 ? m.y : m.z);
+          double sum = 0.0;
+          for (int a = 0; a <= e; ++a)
+            sum += pw[w][d][a] * in[multi_slot(m.x - (d == 0 ? a : 0), m.y - (d == 1 ? a : 0), m.z - (d == 2 ? a : 0))];
+          acc[i] = sum;
+        }
       }
     }
-  }
-  __syncthreads();  // every wave done reading its child
+    __syncthreads();  // every wave done reading `in` (out may alias another pass's input)
+    if (w < k) {
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        const int n = lane + 64 * i;
+        if (n < MS) out[n] = acc[i];
+      }
+    }
+    __syncthreads();
+  };
+  pass(sM[w], sT[w], 0);
+  pass(sT[w], sM[w], 1);
+  pass(sM[w], sT[w], 2);
   if (w < k) {
 #pragma unroll
     for (int i = 0; i < NPL; ++i) {
       const int n = lane + 64 * i;
-      if (n < MS) sM[w][n] = acc[i];
+      if (n < MS) sM[w][n] = sT[w][n];
     }
   }
   __syncthreads();
@@ -367,7 +383,7 @@ __global__ __launch_bounds__(64) void k_l2l_top_wave(int beg, int end, const int
                                                      float* __restrict__ Lf, double* __restrict__ Ltop, int ntop) {
   static_assert(SO == 0 || P < 0, "charges only");
   constexpr int LS = Op<P>::LS, NPL = (LS + 63) / 64;
-  __shared__ double sL[LS];
+  __shared__ double sL[LS], sT[LS];
   __shared__ double pw[3][P + 1];
   const int c = beg + blockIdx.x;
   if (c >= end) return;
@@ -381,22 +397,44 @@ __global__ __launch_bounds__(64) void k_l2l_top_wave(int beg, int end, const int
     for (int e = 1; e <= a; ++e) v = v * u / (double)e;
     pw[d][a] = v;
   }
-  __syncthreads();
+  // the shift L'_n = sum_{j >= n, |j| <= P} L_j u^(j-n)/(j-n)! factors over
+  // the three axes (each pass keeps |j| <= P): one pass per axis, at most
+  // P + 1 terms per slot
   constexpr DegTab<512> tab{};
+  MultiIdx mi[NPL];
+  double acc[NPL];
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) mi[i] = slot_multi(min(lane + 64 * i, LS - 1));
+  __syncthreads();
+  auto pass = [&](const double* in, double* out, int d) {
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const int n = lane + 64 * i;
+      if (n < LS) {
+        const MultiIdx m = mi[i];
+        const int R = P - (m.x + m.y + m.z);
+        double sum = 0.0;
+        for (int a = 0; a <= R; ++a)
+          sum += pw[d][a] * in[multi_slot(m.x + (d == 0 ? a : 0), m.y + (d == 1 ? a : 0), m.z + (d == 2 ? a : 0))];
+        acc[i] = sum;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NPL; ++i) {
+      const int n = lane + 64 * i;
+      if (n < LS) out[n] = acc[i];
+    }
+    __syncthreads();
+  };
+  pass(sL, sT, 0);
+  pass(sT, sL, 1);
+  pass(sL, sT, 2);
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const int n = lane + 64 * i;
     if (n < LS) {
-      const MultiIdx m = slot_multi(n);
-      const int R = P - (m.x + m.y + m.z);
-      double sum = 0.0;
-      for (int ia = 0; ia <= R; ++ia)
-        for (int jb = 0; jb <= R - ia; ++jb) {
-          const double xy = pw[0][ia] * pw[1][jb];
-          for (int kc = 0; kc <= R - ia - jb; ++kc)
-            sum += sL[multi_slot(m.x + ia, m.y + jb, m.z + kc)] * (xy * pw[2][kc]);
-        }
-      const double v = ldexp(sum, -(tab.d[n] + 1));
+      const double v = ldexp(acc[i], -(tab.d[n] + 1));
       const size_t o = (size_t)c * LS + n;
       if (c < ntop) {
         Ltop[o] = (double)Lf[o] + v;
