@@ -42,19 +42,41 @@ __device__ __forceinline__ double rmax_of(double r) { return sqrt_rn((3.0 * r) *
 
 // Sequential mean: the block stages chunks of positions through LDS
 // (coalesced loads), lanes 0..2 add them in index order.
-__global__ void k_mean(const double* __restrict__ pos, long long n, double* __restrict__ out) {
-  constexpr int kChunk = 1024;
-  __shared__ double buf[3 * kChunk];
+// Mean of the positions with the reference's sequential FP64 sum
+// (lanes 0..2 of wave 0 sum x, y, z in particle order, one rounding per
+// addition, so the root centre is bit-exact).  Waves 1..3 stage the next
+// chunk into the other half of a double buffer while wave 0 sums the
+// current one, reading runs of 16 values before their 16 ordered additions.
+__global__ __launch_bounds__(256) void k_mean(const double* __restrict__ pos, long long n, double* __restrict__ out) {
+  constexpr int kChunk = 1024, kRun = 16;
+  __shared__ double buf[2][3 * kChunk];
   double s = 0.0;
-  for (long long base = 0; base < n; base += kChunk) {
+  const int tid = threadIdx.x;
+  auto stage = [&](long long base, int half) {  // waves 1..3
     const int m = (int)(n - base < kChunk ? n - base : kChunk);
-    for (int k = threadIdx.x; k < 3 * m; k += blockDim.x) buf[k] = pos[3 * base + k];
-    __syncthreads();
-    if (threadIdx.x < 3)
-      for (int i = 0; i < m; ++i) s = s + buf[3 * i + threadIdx.x];
-    __syncthreads();
+    for (int k = tid - 64; k < 3 * m; k += 192) buf[half][k] = pos[3 * base + k];
+  };
+  if (tid >= 64 && n > 0) stage(0, 0);
+  __syncthreads();
+  int half = 0;
+  for (long long base = 0; base < n; base += kChunk, half ^= 1) {
+    if (tid >= 64 && base + kChunk < n) stage(base + kChunk, half ^ 1);
+    if (tid < 3) {
+      const int m = (int)(n - base < kChunk ? n - base : kChunk);
+      const double* b = buf[half] + tid;
+      int i = 0;
+      for (; i + kRun <= m; i += kRun) {
+        double v[kRun];
+#pragma unroll
+        for (int u = 0; u < kRun; ++u) v[u] = b[3 * (i + u)];
+#pragma unroll
+        for (int u = 0; u < kRun; ++u) s = s + v[u];
+      }
+      for (; i < m; ++i) s = s + b[3 * i];
+    }
+    __syncthreads();  // this half consumed, the next one staged
   }
-  if (threadIdx.x < 3) out[threadIdx.x] = s / (double)n;
+  if (tid < 3) out[tid] = s / (double)n;
 }
 
 __global__ void k_maxdev(const double* __restrict__ pos, long long n, const double* __restrict__ mean,
