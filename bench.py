@@ -209,6 +209,8 @@ def main():
     ap.add_argument("--theta", type=float, default=None)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--model", default=None)
+    ap.add_argument("--learned-thetas", default="0.5,0.55",
+                    help="opening angles the learned modes may choose from (comma separated; empty: the tree's)")
     ap.add_argument("--m2l-form", default="auto", choices=["auto", "full", "harmonic"],
                     help="M2L operand form of the timed mode (auto: harmonic for the mixed modes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -287,8 +289,12 @@ def main():
     if learned:
         from fmmgen_amd import learned as LN
         params, margin, _ = LN.load(args.model or LN.MODEL_PATH)
-        tree.set_model(params, args.tol, margin)
+        thetas = [float(v) for v in args.learned_thetas.split(",") if v.strip()]
+        tree.set_model(params, args.tol, margin, thetas=thetas or None)
         extra["model_margin"] = margin
+        extra["learned_theta"] = tree.learned_theta
+        print(f"# learned opening angle {tree.learned_theta} (candidates {thetas or [cfg['theta']]})",
+              file=sys.stderr, flush=True)
     ms_per_step, ms_wall, avg = timed(tree, F, mode, args.steps, args.warmup)
     print(f"# {mode}: {ms_per_step:.3f} ms/eval (median), {ms_wall:.3f} (wall mean); phases (ms/eval): "
           + ", ".join(f"{k} {v:.3f}" for k, v in avg.items()), file=sys.stderr, flush=True)
@@ -313,8 +319,9 @@ def main():
     dtype = {"plain": "f64", "f32": "f32", "bf16x3": "bf16x3+f32", "learned": "f32",
              "learned_bf16x3": "bf16x3+f32", "bh": "f64"}[mode]
     workload = (f"{args.config}: N={n} {cfg['dist']}, theta {cfg['theta']}, ncrit {cfg['ncrit']}, "
-                f"source_order {so}, " + (f"learned order 2-{full}, tol {args.tol:g}" if learned
-                                          else f"order {cfg['order']}"))
+                f"source_order {so}, " + (f"learned order 2-{full} and opening angle "
+                                          f"{extra.get('learned_theta', cfg['theta'])}, tol {args.tol:g}"
+                                          if learned else f"order {cfg['order']}"))
     out = {
         "metric": "FMM particle-evals/sec at 1e-6 rel field err, mixed+learned vs FP64 plain",
         "value": value, "unit": "particle-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -53,6 +53,9 @@ class Tree:
         self.outputsize = outputsize(potential, field)
         self.msize = Msize(order, source_order)
         self.lsize = Lsize(order, source_order)
+        self.m2l_form = "auto"
+        self.learned_theta = theta  # opening angle of the learned evaluations
+        self._learned = None        # tree at learned_theta when it differs from theta
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -73,6 +76,9 @@ class Tree:
         if form not in _lib.M2L_FORMS:
             raise ValueError(f"M2L form must be one of {sorted(_lib.M2L_FORMS)}")
         _lib.check(_lib.load().fmm_tree_set_m2l_form(self._h, _lib.M2L_FORMS[form]))
+        self.m2l_form = form
+        if self._learned is not None:
+            self._learned.set_m2l_form(form)
 
     # ----------------------------------------------------------- learned order
     def set_order(self, order):
@@ -81,6 +87,8 @@ class Tree:
         self.order = int(order)
         self.msize = Msize(self.order, self.source_order)
         self.lsize = Lsize(self.order, self.source_order)
+        if self._learned is not None:
+            self._learned.set_order(order)
 
     def nleaves(self):
         return self.info()["nleaves"]
@@ -93,7 +101,9 @@ class Tree:
 
     def leaf_features(self):
         """[nleaves, 12] float32 device tensor of the leaf features (needs
-        one previous evaluation)."""
+        one previous evaluation; those of the learned evaluations' lists)."""
+        if self._learned is not None:
+            return self._learned.leaf_features()
         f = torch.empty(self.nleaves(), _lib.MLP_FEATURES, dtype=torch.float32, device="cuda")
         _lib.check(_lib.load().fmm_tree_leaf_features(self._h, _ptr(f), _stream()))
         return f
@@ -103,6 +113,8 @@ class Tree:
         output scale): rms |F| of FP64 direct sums at the judge's 1024
         stratified targets without the largest 0.1% (needs one previous
         evaluation)."""
+        if self._learned is not None:
+            return self._learned.field_scale()
         v = ctypes.c_double()
         _lib.check(_lib.load().fmm_tree_field_scale(self._h, ctypes.byref(v), _stream()))
         return v.value
@@ -116,13 +128,71 @@ class Tree:
         _lib.check(_lib.load().fmm_tree_leaf_error(self._h, _ptr(F), _ptr(Fref), _ptr(e), _stream()))
         return e
 
-    def set_model(self, params, tol, margin=0.0):
-        """Order network (FP32 parameters, device tensor) of the learned modes."""
+    # Learned opening angle.  Costs of one learned evaluation in seconds per
+    # unit, measured on MI355X at 2^22 blobs (FP32 harmonic-form M2L, FP32
+    # near field): an M2L multiply-add of the harmonic form (full form), a
+    # near-field particle pair, and a (cell, expansion slot) of the upward
+    # and downward passes, which run at the largest order in use.
+    COST_M2L_HARMONIC = 8.0e-14
+    COST_M2L_FULL = 5.9e-14
+    COST_P2P_PAIR = 6.4e-13
+    COST_PASS_SLOT = 3.1e-11
+    MAX_LEARNED_THETA = 0.55  # the network's orders were checked against the judge up to this angle
+
+    def set_model(self, params, tol, margin=0.0, thetas=None):
+        """Order network (FP32 parameters, device tensor) of the learned modes.
+
+        ``thetas``: opening angles (each in (0, MAX_LEARNED_THETA]) the learned
+        modes may choose from for this particle set.  For each one the lists
+        are built and the network predicts the leaf orders; the learned
+        evaluations then run on the lists of the angle whose predicted cost is
+        lowest (the predicted orders' M2L multiply-adds, the near-field pairs
+        and the passes at the largest predicted order, priced by the COST_*
+        figures).  The plain and mixed modes keep the tree's own angle, whose
+        lists are the reference's.  Without ``thetas`` the learned modes use
+        the tree's angle."""
         if params.dtype != torch.float32 or not params.is_cuda or params.numel() != _lib.MLP_PARAMS:
             raise ValueError(f"params must be a float32 CUDA tensor of {_lib.MLP_PARAMS} values")
         _lib.check(_lib.load().fmm_tree_set_model(self._h, _ptr(params.contiguous()), float(tol), float(margin)))
+        self._learned, self.learned_theta = None, self.theta
+        if not thetas:
+            return
+        for th in thetas:
+            if not 0.0 < float(th) <= self.MAX_LEARNED_THETA:
+                raise ValueError(f"learned opening angles must lie in (0, {self.MAX_LEARNED_THETA}]")
+        best_cost, best = self.predicted_learned_cost(), None
+        for th in sorted(set(float(t) for t in thetas)):
+            if th == self.theta:
+                continue
+            t = build_tree(self.pos, self.S, self.n, self.ncrit, self.order, th, self.source_order,
+                           self.potential, self.field)
+            t.set_m2l_form(self.m2l_form)
+            t.set_model(params, tol, margin)
+            c = t.predicted_learned_cost()
+            if c < best_cost:
+                best_cost, best = c, t
+        if best is not None:
+            self._learned, self.learned_theta = best, best.theta
+
+    def predicted_learned_cost(self):
+        """Predicted seconds of one learned evaluation's M2L, near field and
+        upward/downward passes at the orders the network predicted for the
+        current sources (after set_model; used to choose the opening angle)."""
+        inf = self.info()
+        st = self.learned_stats()
+        harm = self.m2l_form != "full" and self.source_order == 0
+        m2l = st["m2l_macs_harmonic"] * self.COST_M2L_HARMONIC if harm else st["m2l_macs"] * self.COST_M2L_FULL
+        qmax = int(self.leaf_orders().max()) if inf["nleaves"] > 0 else 0
+        slots = (qmax + 1) * (qmax + 2) * (qmax + 3) // 6
+        return m2l + inf["p2p_particle_pairs"] * self.COST_P2P_PAIR + inf["ncells"] * slots * self.COST_PASS_SLOT
+
+    def _lt(self):
+        """The tree the learned modes run on."""
+        return self._learned if self._learned is not None else self
 
     def learned_stats(self):
+        if self._learned is not None:
+            return self._learned.learned_stats()
         a = (ctypes.c_double * 11)()
         _lib.check(_lib.load().fmm_tree_learned_stats(self._h, a))
         keys = ("order_max", "judge_failures", "reruns", "judge_rel_err", "judge_targets", "m2l_mac_fraction",
@@ -130,6 +200,8 @@ class Tree:
         return dict(zip(keys, list(a)))
 
     def leaf_orders(self):
+        if self._learned is not None:
+            return self._learned.leaf_orders()
         out = np.zeros(self.nleaves(), dtype=np.int32)
         _lib.check(_lib.load().fmm_tree_leaf_orders(self._h, out.ctypes.data_as(_lib.i_p)))
         return out
@@ -146,19 +218,29 @@ class Tree:
     def timing(self, enable):
         """Switches stream-ordered per-phase hipEvent timing on or off (resets totals)."""
         _lib.check(_lib.load().fmm_timing_enable(self._h, int(bool(enable))))
+        if self._learned is not None:
+            self._learned.timing(enable)
 
     def timing_read(self):
-        """{phase: (total ms, number of timed evaluations)} since timing was enabled."""
+        """{phase: (total ms, number of timed evaluations)} since timing was
+        enabled (both the tree's own evaluations and the learned ones at
+        learned_theta)."""
         ms = (ctypes.c_double * len(_lib.PHASE_NAMES))()
         cnt = (ctypes.c_longlong * len(_lib.PHASE_NAMES))()
         _lib.check(_lib.load().fmm_timing_read(self._h, ms, cnt))
-        return {nm: (ms[i], cnt[i]) for i, nm in enumerate(_lib.PHASE_NAMES) if nm != "unused"}
+        out = {nm: (ms[i], cnt[i]) for i, nm in enumerate(_lib.PHASE_NAMES) if nm != "unused"}
+        if self._learned is not None:
+            for k, (m, c) in self._learned.timing_read().items():
+                out[k] = (out[k][0] + m, out[k][1] + c)
+        return out
 
     def set_sources(self, S):
         """Same geometry, new source strengths (the tree keeps a pointer)."""
         _check_dev(S, "S", sourcesize(self.source_order))
         _lib.check(_lib.load().fmm_tree_set_sources(self._h, _ptr(S)))
         self.S = S
+        if self._learned is not None:
+            self._learned.set_sources(S)
 
     # ----------------------------------------------------------- evaluation
     def compute_field_fmm(self, F, mode="plain", phases=_lib.PHASE_ALL):
@@ -167,6 +249,8 @@ class Tree:
         if F.shape[0] != self.n:
             raise ValueError("F must have one row per particle")
         m = _lib.MODES[mode] if isinstance(mode, str) else int(mode)
+        if self._learned is not None and m in (_lib.MODE_LEARNED_F32, _lib.MODE_LEARNED_BF16X3):
+            return self._learned.compute_field_fmm(F, m, phases)
         _lib.check(_lib.load().fmm_evaluate_phases(self._h, m, int(phases), _ptr(F), _stream()))
         return F
 

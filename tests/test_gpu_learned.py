@@ -308,6 +308,53 @@ def test_learned_full_form_m2l_meets_tolerance(gpu, mode):
 
 
 @pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
+def test_learned_opening_angle_choice(gpu):
+    """set_model with candidate opening angles: the learned evaluations run
+    on the lists of the angle with the lowest predicted cost (the same
+    result as a tree built at that angle) and meet the tolerance over every
+    target; the plain mode keeps the tree's own angle; angles beyond the
+    network's checked range are refused."""
+    from fmmgen_amd import build_tree, direct_sum
+    from fmmgen_amd import learned as LN
+    tol, n = 1e-6, 1 << 17
+    pos = _positions("blobs", n, 41)
+    q = np.random.default_rng(42).uniform(-1, 1, (n, 1))
+    dpos, dq = _dev(pos), _dev(q)
+    tg = _all_targets(n)
+    Fd = direct_sum(dpos, dq, torch.from_numpy(tg).cuda()).cpu().numpy()
+    params, margin, _ = LN.load(MODEL)
+    t = build_tree(dpos, dq, n, 64, 12, 0.5)
+    t.set_model(params, tol, margin, thetas=(0.5, 0.55))
+    costs = {}
+    for th in (0.5, 0.55):
+        u = build_tree(dpos, dq, n, 64, 12, th)
+        u.set_model(params, tol, margin)
+        costs[th] = u.predicted_learned_cost()
+    _log(f"  predicted costs {costs}, chosen {t.learned_theta}")
+    assert t.learned_theta == min(costs, key=costs.get)
+    F = torch.zeros(n, 4, dtype=torch.float64, device="cuda")
+    t.compute_field_fmm(F, mode="learned")
+    e, _ = _errors(F.cpu().numpy(), Fd)
+    assert e <= tol
+    u = build_tree(dpos, dq, n, 64, 12, t.learned_theta)
+    u.set_model(params, tol, margin)
+    F2 = torch.zeros_like(F)
+    u.compute_field_fmm(F2, mode="learned")
+    assert _rel_err(F.cpu().numpy(), F2.cpu().numpy(), np.arange(n)) < 1e-6
+    # plain mode on the reference angle's lists
+    t.set_order(8)
+    Fp = torch.zeros_like(F)
+    t.compute_field_fmm(Fp, mode="plain")
+    v = build_tree(dpos, dq, n, 64, 8, 0.5)
+    Fv = torch.zeros_like(F)
+    v.compute_field_fmm(Fv, mode="plain")
+    # (the M2L chunks of a tile add atomically: equal to rounding)
+    assert _rel_err(Fp.cpu().numpy(), Fv.cpu().numpy(), np.arange(n)) < 1e-12
+    with pytest.raises(ValueError):
+        t.set_model(params, tol, margin, thetas=(0.6,))
+
+
+@pytest.mark.skipif(not os.path.exists(MODEL), reason="no trained order model in the tree")
 def test_learned_full_size_blobs_meets_tolerance(gpu):
     """The headline workload at full size: N = 2^22 blobs, ncrit 64,
     theta 0.5, orders up to 12, tolerance 1e-6, checked at 4096 targets
