@@ -148,6 +148,24 @@ __host__ __device__ constexpr int multi_slot(int x, int y, int z) {
   const int d = x + y + z, a = d - x;
   return nterms(d - 1) + a * (a + 1) / 2 + (d - x - y);
 }
+// Multi-index of every graded slot up to order 15, packed x | y << 8 |
+// z << 16, for slot indices known only at run time (one load instead of
+// slot_multi's two searches).
+struct SlotMultiTab {
+  unsigned v[nterms(15)];
+  constexpr SlotMultiTab() : v{} {
+    for (int j = 0; j < nterms(15); ++j) {
+      const MultiIdx m = slot_multi(j);
+      v[j] = (unsigned)m.x | ((unsigned)m.y << 8) | ((unsigned)m.z << 16);
+    }
+  }
+};
+__device__ __forceinline__ MultiIdx slot_multi_rt(int j) {
+  constexpr SlotMultiTab tab{};
+  const unsigned u = tab.v[j];
+  return MultiIdx{(int)(u & 255u), (int)((u >> 8) & 255u), (int)(u >> 16)};
+}
+
 // The same for the harmonic layout (n_z <= 1): within degree d, slot
 // d^2 + r holds a = n_y + n_z = (r + 1) / 2 and n_z = (r + 1) % 2 (r > 0).
 __host__ __device__ constexpr MultiIdx harm_multi(int j) {

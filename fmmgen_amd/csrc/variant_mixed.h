@@ -117,7 +117,7 @@ __global__ __launch_bounds__(64) void k_p2m_wave(int nleaves, const int* __restr
 #pragma unroll
   for (int i = 0; i < NPL; ++i) {
     const int sl = lane + 64 * i;
-    const MultiIdx m = slot_multi(sl < MS ? sl : 0);
+    const MultiIdx m = slot_multi_rt(sl < MS ? sl : 0);
     ea[i] = m.x;
     eb[i] = P + 1 + m.y;
     ec[i] = 2 * (P + 1) + m.z;
@@ -330,7 +330,7 @@ __global__ __launch_bounds__(512) void k_m2m_top_wave(int beg, int end, const in
   // axes: one pass per axis, each a sum of at most P + 1 terms per slot
   MultiIdx mi[NPL];
 #pragma unroll
-  for (int i = 0; i < NPL; ++i) mi[i] = slot_multi(min(lane + 64 * i, MS - 1));
+  for (int i = 0; i < NPL; ++i) mi[i] = slot_multi_rt(min(lane + 64 * i, MS - 1));
   __syncthreads();
   auto pass = [&](const double* in, double* out, int d) {
     if (w < k) {
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64) void k_l2l_top_wave(int beg, int end, const int
   MultiIdx mi[NPL];
   double acc[NPL];
 #pragma unroll
-  for (int i = 0; i < NPL; ++i) mi[i] = slot_multi(min(lane + 64 * i, LS - 1));
+  for (int i = 0; i < NPL; ++i) mi[i] = slot_multi_rt(min(lane + 64 * i, LS - 1));
   __syncthreads();
   auto pass = [&](const double* in, double* out, int d) {
 #pragma unroll
