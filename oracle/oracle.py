@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` use this module, as the checker; the product never imports it.
+``bench.py`` use this module, to compare against; the product never imports it.
 The library is a C++ restatement of the reference example tree code
 (example/tree.cpp, example/calculate.cpp) built by ``oracle/Makefile``.
 """
