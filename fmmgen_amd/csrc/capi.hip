@@ -491,6 +491,7 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
       build_m2l_schedule(t, nullptr);
       FMM_HIP(hipStreamSynchronize(nullptr));
     }
+    t.lstats_ready = false;  // the order statistics now describe the predicted orders
   });
 }
 

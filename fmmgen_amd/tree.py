@@ -150,7 +150,8 @@ class Tree:
         and the passes at the largest predicted order, priced by the COST_*
         figures).  The plain and mixed modes keep the tree's own angle, whose
         lists are the reference's.  Without ``thetas`` the learned modes use
-        the tree's angle."""
+        the tree's angle.  The candidate lists are built from the ``pos``
+        tensor given to build_tree, which must hold the same positions then."""
         if params.dtype != torch.float32 or not params.is_cuda or params.numel() != _lib.MLP_PARAMS:
             raise ValueError(f"params must be a float32 CUDA tensor of {_lib.MLP_PARAMS} values")
         _lib.check(_lib.load().fmm_tree_set_model(self._h, _ptr(params.contiguous()), float(tol), float(margin)))

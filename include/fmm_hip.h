@@ -246,7 +246,10 @@ int fmm_tree_set_model(fmm_tree_t tree, const float* d_params, double tol, doubl
  * C(q + 6, 6) at the pair's order), predicted error of the chosen orders
  * relative to F_s, the field scale F_s, M2L multiply-adds at the target
  * cells' own orders (without the tile maximum), M2L useful multiply-adds
- * in the harmonic form (sum over pairs of sum_n (2n+1)(q-n+1)^2). */
+ * in the harmonic form (sum over pairs of sum_n (2n+1)(q-n+1)^2).  After
+ * fmm_tree_set_model and before the next learned evaluation, entries 5, 6,
+ * 9 and 10 describe the orders set_model predicted (fmm_tree_leaf_orders
+ * likewise); the others keep the last evaluation's values. */
 #define FMM_LEARNED_STATS_LEN 11
 int fmm_tree_learned_stats(fmm_tree_t tree, double* h_stats);
 /* Per-leaf orders of the last learned evaluation: h_order [nleaves]. */
