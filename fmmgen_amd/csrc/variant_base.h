@@ -66,33 +66,34 @@ __global__ void k_op_p2p(int n, const double* __restrict__ xyz, const double* __
 // sorted slots.  A block takes kJudgeTile targets (held in registers, so
 // each source read serves kJudgeTile pairs) against one slice of
 // kJudgeSlice sources, and adds its partial sums atomically into the
-// zero-filled Fd[J][OUT].
+// zero-filled Fd[J][OUT] (for charges with the field: the field components;
+// the potential component stays zero).
 constexpr int kJudgeSlice = 16384;
 constexpr int kJudgeTile = 8;
 
-// 1/sqrt(x) in FP64 for x > 0 (0 for x = 0): the hardware estimate y0
-// refined by the second-order series of (1 - e)^(-1/2), e = 1 - x y0^2:
-// y = y0 + y0 e (1/2 + 3e/8), which cubes the estimate's relative error.
-__device__ __forceinline__ double judge_rsqrt(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
-  const double e = fma(-x * y, y, 1.0);
-  const double r = fma(y * e, fma(0.375, e, 0.5), y);
-  return x > 0.0 ? r : 0.0;
-}
-
-// One charge pair of the judge (potential and field): the terms of the
-// example's P2P (example/calculate.cpp:24-40) with the reciprocal square
-// root above; a coincident pair (the target itself) adds nothing.
+// One charge pair of the judge's field, q r^-3 (dx, dy, dz) (the field terms
+// of the example's P2P, example/calculate.cpp:24-40; the judge's check and
+// the field scale read only the field, so the potential is not summed).
+// r^-3 = x^(-3/2), x = r^2, from the hardware estimate y of x^(-1/2):
+// x^(-3/2) = y^3 (1 - e)^(-3/2) with e = 1 - x y^2, and the series
+// 1 + 3e/2 + 15e^2/8 cubes the estimate's relative error.  A coincident
+// pair (the target itself, x = 0) adds nothing: the factor's high word is
+// cleared, which leaves a zero or subnormal factor times a zero
+// displacement (one select instead of two for the whole FP64 value).
 template <int NOUT>
 __device__ __forceinline__ void judge_pair(double dx, double dy, double dz, double q, double (&acc)[NOUT]) {
-  const double ri = judge_rsqrt(fma(dx, dx, fma(dy, dy, dz * dz)));
-  const double qr = q * ri;
-  const double qr3 = qr * ri * ri;
+  const double x = fma(dx, dx, fma(dy, dy, dz * dz));
+  const double y = __builtin_amdgcn_rsq(x);
+  const double y2 = y * y;
+  const double e = fma(-x, y2, 1.0);
+  const double qy3 = q * (y2 * y);
+  const double t = qy3 * e;
+  double s = fma(t * e, 1.875, fma(t, 1.5, qy3));
+  s = __hiloint2double(x > 0.0 ? __double2hiint(s) : 0, __double2loint(s));
   constexpr int F0 = NOUT == 4 ? 1 : 0;
-  if constexpr (NOUT == 4) acc[0] += qr;
-  acc[F0] = fma(qr3, dx, acc[F0]);
-  acc[F0 + 1] = fma(qr3, dy, acc[F0 + 1]);
-  acc[F0 + 2] = fma(qr3, dz, acc[F0 + 2]);
+  acc[F0] = fma(s, dx, acc[F0]);
+  acc[F0 + 1] = fma(s, dy, acc[F0 + 1]);
+  acc[F0 + 2] = fma(s, dz, acc[F0 + 2]);
 }
 __global__ void __launch_bounds__(256) k_judge_direct(int n, const double* __restrict__ px,
                                                       const double* __restrict__ py, const double* __restrict__ pz,
