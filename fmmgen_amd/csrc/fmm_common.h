@@ -233,6 +233,9 @@ struct Tree {
   int ml_order = -1;
   // mixed precision: normalised FP32 expansions, FP64 copies of the top levels
   DevBuf<float> Mf, Lf;
+  // FP32 M2M: each child's multipole translated to its parent's centre, one
+  // level's children at a time ([cells of the largest level * Msize])
+  DevBuf<float> Mpart;
   // M2L operand form (kM2LAuto / kM2LFull / kM2LHarmonic) and the folded
   // multipoles and harmonic local rows of a harmonic-layout M2L
   int m2l_form = 0;

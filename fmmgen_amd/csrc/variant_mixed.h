@@ -189,6 +189,58 @@ __global__ __launch_bounds__(64) void k_m2m_mixed(int beg, int end, const int* _
   });
 }
 
+// FP32 M2M of one level in two passes.  First one thread per (child, slot
+// chunk) translates the child's multipole to its parent's centre into the
+// child's row of Mpart (rows from cbeg, the level's first child); then
+// k_m2m_sum adds each parent's rows in child order.  Against one thread per
+// parent looping over its children (k_m2m_mixed) this gives eight times the
+// threads and one child's chain of loads per thread, which pays on levels of
+// few cells, bound by that chain's latency (at order 12 on 2^22 blobs: a
+// level of 1344 children 89 -> 24 us); on large levels, bound by the
+// generated bodies' issue at two waves per SIMD, the second pass's traffic
+// makes it slower (88832 children 436 -> 482 us).
+constexpr int kM2MChildMax = 32768;
+template <int P>
+__global__ __launch_bounds__(64) void k_m2m_child(int cbeg, int cend, const int* __restrict__ parent,
+                                                  const double* __restrict__ cx, const double* __restrict__ cy,
+                                                  const double* __restrict__ cz, const double* __restrict__ cr,
+                                                  const float* __restrict__ Mf, float* __restrict__ Mpart) {
+  constexpr int MS = Op<P>::MS, NCH = (MS + kChunk - 1) / kChunk, OFF = nterms(SO - 1);
+  const int c = cbeg + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cend) return;
+  const int p = parent[c];
+  const double ri = 1.0 / cr[p];
+  dispatch_chunk<NCH>(blockIdx.y, [&](auto C) {
+    constexpr int LO = C * kChunk, HI = (LO + kChunk < MS) ? LO + kChunk : MS;
+    ChunkAcc<float, LO, HI> acc;
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) acc.v[j] = 0.f;
+    fmmgen_ops::M2M_chunk<P, LO / kChunk, float>((float)((cx[p] - cx[c]) * ri), (float)((cy[p] - cy[c]) * ri),
+                                                 (float)((cz[p] - cz[c]) * ri),
+                                                 HalfScaledIn<float, OFF>{Mf + (size_t)c * MS}, acc);
+#pragma unroll
+    for (int j = 0; j < HI - LO; ++j) Mpart[(size_t)(c - cbeg) * MS + LO + j] = acc.v[j];
+  });
+}
+
+// Second pass: one thread per (parent, slot), the children's rows summed in
+// child order.
+template <int P>
+__global__ void k_m2m_sum(int beg, int end, int cbeg, const int* __restrict__ first_child,
+                          const int* __restrict__ nchild, const float* __restrict__ Mpart,
+                          float* __restrict__ Mf) {
+  constexpr int MS = Op<P>::MS;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= (long long)(end - beg) * MS) return;
+  const int p = beg + (int)(i / MS), j = (int)(i % MS);
+  const int k = nchild[p];
+  if (k == 0) return;
+  const float* row = Mpart + (size_t)(first_child[p] - cbeg) * MS + j;
+  float s = 0.f;
+  for (int c = 0; c < k; ++c) s += row[(size_t)c * MS];
+  Mf[(size_t)p * MS + j] = s;
+}
+
 // FP64 M2M of a top-level parent; children below the top read the FP32
 // array.  The result is kept in FP64 and rounded into the FP32 array for M2L.
 template <int P>
@@ -512,6 +564,15 @@ void far_field_mixed(Tree& t, int mode, int mask, hipStream_t st) {
   }
   if (mask & kPhaseM2M) {
     PhaseTimer pt(t, kTM2M, st);
+    // levels whose children number at most kM2MChildMax run in two passes
+    // (k_m2m_child, k_m2m_sum), larger ones with one thread per parent;
+    // Mpart is sized once for the largest such child level, before any launch
+    int maxc = 1;
+    for (int l = t.depth - 2; l >= 0; --l) {
+      const int nch = t.level_off[l + 2] - t.level_off[l + 1];
+      if (!(top64 && l < kTopLevels) && nch <= kM2MChildMax) maxc = std::max(maxc, nch);
+    }
+    t.Mpart.alloc((size_t)maxc * MS);
     for (int l = t.depth - 2; l >= 0; --l) {
       const int b = t.level_off[l], e = t.level_off[l + 1];
       const dim3 g(nblk(e - b, 64), (MS + kChunk - 1) / kChunk);
@@ -523,9 +584,16 @@ void far_field_mixed(Tree& t, int mode, int mask, hipStream_t st) {
         else
           k_m2m_top<P><<<g, 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(),
                                          t.cz.get(), t.cr.get(), t.Mf.get(), t.Mtop.get(), ntop);
-      } else
+      } else if (t.level_off[l + 2] - e > kM2MChildMax) {
         k_m2m_mixed<P><<<g, 64, 0, st>>>(b, e, t.first_child.get(), t.nchild.get(), t.cx.get(), t.cy.get(),
                                          t.cz.get(), t.cr.get(), t.Mf.get());
+      } else {
+        const int cb = e, ce = t.level_off[l + 2];  // the level's children: the next level
+        k_m2m_child<P><<<dim3(nblk(ce - cb, 64), (MS + kChunk - 1) / kChunk), 64, 0, st>>>(
+            cb, ce, t.parent.get(), t.cx.get(), t.cy.get(), t.cz.get(), t.cr.get(), t.Mf.get(), t.Mpart.get());
+        k_m2m_sum<P><<<nblk((long long)(e - b) * MS, 256), 256, 0, st>>>(b, e, cb, t.first_child.get(),
+                                                                         t.nchild.get(), t.Mpart.get(), t.Mf.get());
+      }
     }
   }
   if (mask & kPhaseM2L) {
