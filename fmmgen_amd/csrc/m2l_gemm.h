@@ -693,8 +693,7 @@ __global__ __launch_bounds__(256) void k_harm_unfold(int nc, const T* __restrict
       for (int n = lane; n < H::GS; n += 64) {
         double v = 0.0;
         for (int t = s_off[n]; t < s_off[n + 1]; ++t) v += (double)s_w[t] * (double)s_row[w][s_src[t]];
-        T* o = L + (size_t)cell * ls + n;
-        *o = (T)((double)*o + v);
+        L[(size_t)cell * ls + n] = (T)v;  // the first writer of L: every slot of every cell
       }
   }
 }

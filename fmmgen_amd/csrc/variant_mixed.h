@@ -549,8 +549,15 @@ void far_field_mixed(Tree& t, int mode, int mask, hipStream_t st) {
   t.Lf.alloc((size_t)nc * LS);
   t.Mtop.alloc((size_t)std::max(ntop, 1) * MS);
   t.Ltop.alloc((size_t)std::max(ntop, 1) * LS);
-  FMM_HIP(hipMemsetAsync(t.Mf.get(), 0, sizeof(float) * nc * MS, st));
-  FMM_HIP(hipMemsetAsync(t.Lf.get(), 0, sizeof(float) * nc * LS, st));
+  // P2M writes every slot of every leaf and M2M of every parent; the
+  // harmonic form's unfold stores every slot of every local expansion.  The
+  // clears are needed only when a phase is masked out, for the full form
+  // (its M2L adds into Lf) and when there are no M2L groups.
+  if (!((mask & kPhaseP2M) && (mask & kPhaseM2M)))
+    FMM_HIP(hipMemsetAsync(t.Mf.get(), 0, sizeof(float) * nc * MS, st));
+  bool unfold_all = false;
+  if constexpr (SO == 0) unfold_all = (mask & kPhaseM2L) && t.sched.ngroups > 0 && m2l_harmonic(t, mode);
+  if (!unfold_all) FMM_HIP(hipMemsetAsync(t.Lf.get(), 0, sizeof(float) * nc * LS, st));
   if (mask & kPhaseP2M) {
     PhaseTimer pt(t, kTP2M, st);
     if constexpr (SO == 0)
