@@ -149,6 +149,9 @@ int fmm_tree_destroy(fmm_tree_t tree) {
         (void)hipEventDestroy(e.second.second);
       }
       for (auto e : tree->t.event_pool) (void)hipEventDestroy(e);
+      if (tree->t.side) (void)hipStreamDestroy(tree->t.side);
+      if (tree->t.side_fork) (void)hipEventDestroy(tree->t.side_fork);
+      if (tree->t.side_join) (void)hipEventDestroy(tree->t.side_join);
     }
     delete tree;
   });

@@ -214,6 +214,10 @@ struct Tree {
   // M2L by tile-order classes (set by the learned evaluation): called by
   // the far field of order P after the D table is built at order P
   void (*m2l_classes)(Tree&, int mode, int P, hipStream_t) = nullptr;
+  // second stream of the per-order M2L class launches (default priority,
+  // created on first use) and its fork / join events
+  hipStream_t side = nullptr;
+  hipEvent_t side_fork = nullptr, side_join = nullptr;
   DevBuf<_Float16> mlp_half;       // network parameters in FP16
   bool has_model = false;
   double tol = 1e-6, margin = 0.0;

@@ -553,12 +553,26 @@ void gather_sources(Tree& t, hipStream_t st) {
 // order (the ranges read back with the far field's order), each group
 // launched with the kernel of its own order on the expansions of the far
 // field's order P (row strides of order P).
+// The classes alternate between the evaluation stream and a second stream
+// (they add into disjoint or atomically updated local rows), so that one
+// class's last blocks overlap the next class's first.
 void m2l_by_class(Tree& t, int mode, int P, hipStream_t st) {
   const int* start = t.class_start_h;
+  if (!t.side) {
+    FMM_HIP(hipStreamCreateWithFlags(&t.side, hipStreamNonBlocking));
+    FMM_HIP(hipEventCreateWithFlags(&t.side_fork, hipEventDisableTiming));
+    FMM_HIP(hipEventCreateWithFlags(&t.side_join, hipEventDisableTiming));
+  }
+  FMM_HIP(hipEventRecord(t.side_fork, st));
+  FMM_HIP(hipStreamWaitEvent(t.side, t.side_fork, 0));
+  int k = 0;
   for (int q = PMIN; q <= P && q < M2LSchedule::kMaxOrder; ++q) {
     const int nch = start[q + 1] - start[q];
-    if (nch > 0) OrderTable<PMIN>::m2l(q, t, mode, t.sched.class_chunks.get() + start[q], nch, P, st);
+    if (nch > 0)
+      OrderTable<PMIN>::m2l(q, t, mode, t.sched.class_chunks.get() + start[q], nch, P, (k++ & 1) ? t.side : st);
   }
+  FMM_HIP(hipEventRecord(t.side_join, t.side));
+  FMM_HIP(hipStreamWaitEvent(st, t.side_join, 0));
 }
 
 constexpr int kJudgeTargets = 1024;
