@@ -9,7 +9,9 @@
  * synchronises once, to launch its far field at the largest selected order
  * (and once more per judge check: the judge's verdict decides a rerun); the
  * first evaluation of a tree at each order synchronises once, uploading
- * that order's M2L index table.
+ * that order's M2L index table.  A learned evaluation runs half of its M2L
+ * order classes on a second stream of the tree, forked from and joined to
+ * the evaluation stream by events: the work stays ordered on that stream.
  *
  * Reference counterparts (paths in the rpep/fmmgen tree):
  *   fmm_tree_build        build_tree(pos, S, n, ncrit, order, theta)
