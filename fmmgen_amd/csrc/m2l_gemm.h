@@ -666,8 +666,9 @@ __global__ __launch_bounds__(256) void k_harm_fold(int nc, const T* __restrict__
   }
 }
 
-// Unfold: L[cell][n] += the graded local expansion rebuilt from the harmonic
-// rows Lh[cell] (stride hs).
+// Unfold: L[cell][n] = the graded local expansion rebuilt from the harmonic
+// rows Lh[cell] (stride hs), stored into every slot of every cell (the M2L
+// is the first writer of L in an evaluation).
 template <int P, typename T>
 __global__ __launch_bounds__(256) void k_harm_unfold(int nc, const T* __restrict__ Lh, int hs, T* __restrict__ L,
                                                      int ls) {

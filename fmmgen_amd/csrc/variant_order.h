@@ -134,7 +134,7 @@ const unsigned* ensure_index_table(Tree& t, hipStream_t st) {
 
 // Harmonic-layout M2L around the GEMM: fold every cell's multipole (M,
 // graded, stride ms) into Mh, clear Lh (stride of order ps), then, after
-// the GEMM, add the unfolded rows into L (graded, stride ls).
+// the GEMM, store the unfolded rows into L (graded, stride ls).
 template <int PS, typename T>
 void harm_fold(Tree& t, const T* M, int ms, DevBuf<T>& Mh, DevBuf<T>& Lh, hipStream_t st) {
   const int nc = t.ncells, hs = harm_size(PS);
